@@ -1,0 +1,214 @@
+"""Benchmark: gate HBM throughput of the HIP state-vector runtime on one MI355X.
+
+Workload (one step): the 2000-gate random circuit on 30 qubits, complex64 — gate kinds drawn
+uniformly from {U3, H, Rz, CX, CCX} with seed 2 (qgate_amd.workloads.random_circuit) —
+applied to a state vector already resident in HBM, through the runtime-module calls the
+simulator front-end makes for it (processor.apply_gate / apply_controlled_gate). After the
+timed steps the same state is used for the end-of-circuit work of the workload (sampling
+pool + 2^20 draws, per-qubit measurement), timed separately and reported as extra fields.
+
+value = algorithmic HBM bytes of the step / step time (GB/s). Algorithmic bytes per gate:
+uncontrolled 2x2 gate 2 S 2^n, c-controlled 2 S 2^(n-c), diagonal gates with m00 = 1
+(only the |1> half changes) S 2^n / 2^c; S = 8 (complex64).
+
+roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, timed with
+HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
+launch time, against 8 TB/s.
+
+cpu_baseline: the reference's own C++ CPU runtime (oracle/_ref, or the C restatement when
+that is not built) applying the first gates of the same circuit to a 30-qubit complex64
+state in host memory, for about 15 s of work, all host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = 'gate HBM GB/s (% peak) at 30q c64; adjoint grad time / forward time, 24q'
+
+
+def gate_bytes(kind, n, amp):
+    """Algorithmic HBM bytes of one gate of the workload."""
+    if kind in ('U3', 'H', 'Rz'):
+        return 2 * amp * (1 << n)
+    if kind == 'CX':
+        return 2 * amp * (1 << (n - 1))
+    if kind == 'CCX':
+        return 2 * amp * (1 << (n - 2))
+    raise ValueError(kind)
+
+
+class Runner:
+    """Applies a workload gate list through a runtime module's processor, lane i = qubit i."""
+
+    def __init__(self, module, n, dtype):
+        self.module = module
+        self.qs = module.create_qubit_states(dtype)
+        self.proc = self.qs.processor
+        self.proc.initialize_qubit_states(self.qs, n)
+        self.proc.reset_qubit_states(self.qs)
+        from qgate_amd.model import gate_type as gt
+        self.H = gt.H()
+        self.X = gt.X()
+        self.gt = gt
+
+    def apply(self, gates):
+        proc, qs, gt = self.proc, self.qs, self.gt
+        for kind, qubits, angles in gates:
+            if kind == 'U3':
+                proc.apply_gate(gt.U(*angles), False, qs, qubits[0])
+            elif kind == 'H':
+                proc.apply_gate(self.H, False, qs, qubits[0])
+            elif kind == 'Rz':
+                proc.apply_gate(gt.RZ(angles[0]), False, qs, qubits[0])
+            else:
+                proc.apply_controlled_gate(self.X, False, qs, qubits[:-1], qubits[-1])
+
+    def sync(self):
+        self.proc.synchronize()
+
+    def delete(self):
+        self.qs.delete()
+
+
+def cpu_baseline(gates, n, amp, budget_s=15.0):
+    from oracle import cpuruntime
+    if cpuruntime.have_reference():
+        module, kind = cpuruntime.reference(), 'reference'
+    else:
+        if not os.path.exists(cpuruntime.PORT_LIB):
+            cpuruntime.build(reference=False)
+        module, kind = cpuruntime.port(), 'port'
+    cores = module.runtime.L.qgo_num_workers()
+    r = Runner(module, n, np.float32)
+    done, nbytes = 0, 0
+    t0 = time.perf_counter()
+    while done < len(gates) and time.perf_counter() - t0 < budget_s:
+        r.apply(gates[done:done + 1])
+        nbytes += gate_bytes(gates[done][0], n, amp)
+        done += 1
+    dt = time.perf_counter() - t0
+    r.delete()
+    return {'value': round(nbytes / dt / 1e9, 3), 'unit': 'GB/s', 'cores': cores, 'kind': kind,
+            'sample': 'first {} of the 2000 gates (30 qubits, complex64, host memory), {:.1f} s, '
+                      '{:.3f} s/gate'.format(done, dt, dt / max(done, 1))}
+
+
+def roofline(runner, n, amp, reps=4):
+    """Mean launch time of the uncontrolled 2x2 pair kernel over every target bit."""
+    from qgate_amd.simulator import hipruntime
+    mat = hipruntime.gate_matrix(runner.gt.U(0.3, 1.1, 2.3))
+    per_target = []
+    for t in range(n):
+        per_target.append(runner.proc.profile_gate(mat, runner.qs, [], t, reps))
+    avg_ms = float(np.mean(per_target))
+    launch_bytes = 2 * amp * (1 << n)
+    achieved = launch_bytes / (avg_ms * 1e-3) / 1e9
+    split = {}
+    for name, rng in (('t<6', range(0, 6)), ('6<=t<13', range(6, 13)), ('t>=13', range(13, n))):
+        ms = float(np.mean([per_target[t] for t in rng]))
+        split[name] = round(launch_bytes / (ms * 1e-3) / 1e9, 1)
+    return {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+            'kernel': 'k_pairs (uncontrolled 2x2, complex64, 30 qubits)',
+            'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
+            'by_target_GBs': split}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--qubits', type=int, default=30)
+    ap.add_argument('--gates', type=int, default=2000)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: the state-vector backend does not distribute qubits '
+              'across GPUs (got --gpus {}).'.format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+
+    from qgate_amd import workloads
+    from qgate_amd.simulator import hipruntime
+
+    n, amp = args.qubits, 8
+    gates = workloads.random_gate_list(n, args.gates, seed=2)
+    step_bytes = sum(gate_bytes(k, n, amp) for k, _, _ in gates)
+    info = hipruntime.device_info()
+
+    r = Runner(hipruntime, n, np.float32)
+    print('[bench] device {} ({} CUs), {} qubits complex64, {} gates, {:.1f} GB per step'
+          .format(info['device_id'], info['n_cus'], n, len(gates), step_bytes / 1e9), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        r.apply(gates)
+    r.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r.apply(gates)
+        r.sync()
+        print('[bench] step {} done, {:.3f} s'.format(i, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    step_s = (t1 - t0) / args.steps
+    value = step_bytes / step_s / 1e9
+
+    # end-of-circuit work of the workload on the same state: pool + 2^20 draws, measures
+    from qgate_amd.simulator.lanes import Lane
+    getter = hipruntime.create_qubits_states_getter(np.float32)
+    lanes = []
+    for i in range(n):
+        ln = Lane(None, i)
+        ln.set_external(i)
+        lanes.append(ln)
+    u = np.random.RandomState(1002).random_sample(1 << 20)
+    t2 = time.perf_counter()
+    pool = getter.create_sampling_pool(list(range(n)), n, 0, [(r.qs, lanes)], [])
+    t3 = time.perf_counter()
+    pool.sample(len(u), u)
+    t4 = time.perf_counter()
+    pool.delete()
+    um = np.random.RandomState(1003).random_sample(n)
+    for lane in range(n):
+        p = r.proc.calc_probability(r.qs, lane)
+        r.proc.decohere(0 if um[lane] < p else 1, p, r.qs, lane)
+    r.sync()
+    t5 = time.perf_counter()
+
+    roof = roofline(r, n, amp)
+    r.delete()
+    print('[bench] roofline: {:.1f} GB/s'.format(roof['achieved']), file=sys.stderr, flush=True)
+
+    result = {
+        'metric': METRIC, 'value': round(value, 1), 'unit': 'GB/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(step_s * 1e3, 2),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'complex64 (f32)', 'data': 'synthetic',
+        'config': {'workload': 'random circuit, {} qubits, {} gates from {{U3,H,Rz,CX,CCX}}, '
+                               'seed 2, complex64, fusion off'.format(n, len(gates)),
+                   'qubits': n, 'gates': len(gates), 'fusion': 'off',
+                   'parallelism': 'single GPU'},
+        'frac_of_hbm_peak': round(value / HBM_PEAK_GBS, 4),
+        'gates_per_s': round(len(gates) / step_s, 1),
+        'pool_build_ms': round((t3 - t2) * 1e3, 2),
+        'pool_2p20_draws_ms': round((t4 - t3) * 1e3, 2),
+        'measure_all_ms': round((t5 - t4) * 1e3, 2),
+        'roofline': roof,
+    }
+    if not args.no_cpu:
+        print('[bench] CPU baseline ...', file=sys.stderr, flush=True)
+        result['cpu_baseline'] = cpu_baseline(gates, n, amp)
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,187 @@
+/*
+ * C ABI of the qgate HIP runtime (libqgate_hip.so): the MI355X state-vector backend behind
+ * qgate's runtime-module protocol.
+ *
+ * The entry points are the ones the reference binds from Python through its CPython glue
+ * module and backend factories (qgate/simulator/src/glue.cpp:605-631, cpuext.cpp:73-79,
+ * cudaext.cpp:127-136) over the native interfaces of qgate/simulator/src/Interfaces.h:7-81,
+ * restated with plain C types: opaque handles instead of numpy.uint64-boxed pointers, and
+ * caller-owned buffers with explicit sizes instead of ndarrays.
+ *
+ * Conventions
+ *   - Every function returns QGHIP_OK (0) on success, QGHIP_ERROR (-1) on a runtime error or
+ *     QGHIP_EVALUE (-2) on an invalid argument; qghip_last_error() returns the message of the
+ *     last failure on the calling thread. No C++ exception crosses the boundary (the
+ *     reference glue lets them terminate the process).
+ *   - prec: QGHIP_FP64 (complex128 amplitudes) or QGHIP_FP32 (complex64 amplitudes).
+ *   - Lanes are local qubit indices of one state vector: bit `lane` of the amplitude index.
+ *   - All device work of the library is ordered on one HIP stream; calls that return host
+ *     data (probabilities, states, samples) synchronise it.
+ *   - Not thread-safe per handle; calls on different handles may come from one thread at a
+ *     time per process (the reference holds the GIL for every call).
+ */
+#ifndef QGATE_HIP_H
+#define QGATE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QGHIP_OK = 0, QGHIP_ERROR = -1, QGHIP_EVALUE = -2 };
+enum { QGHIP_FP64 = 0, QGHIP_FP32 = 1 };
+enum { QGHIP_MATHOP_NULL = 0, QGHIP_MATHOP_PROB = 1 };
+
+typedef struct qghip_qubit_states qghip_qubit_states;
+typedef struct qghip_qubit_processor qghip_qubit_processor;
+typedef struct qghip_qubits_states_getter qghip_qubits_states_getter;
+typedef struct qghip_sampling_pool qghip_sampling_pool;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *qghip_last_error(void);
+
+/* ---- module / device ------------------------------------------------------------------ */
+
+/* Select the GPU and create the library stream; replaces devices_initialize
+ * (cudaext.cpp:32-58, called from cudaruntime.py:74-76 module_init). One device only. */
+int qghip_devices_initialize(int device_id);
+/* Free cached device memory and the stream; replaces devices_clear (cudaext.cpp:60-68,
+ * cudaruntime.py:78-84 module_finalize). */
+int qghip_devices_clear(void);
+/* Device id, compute-unit count and HBM size of the initialised device. */
+int qghip_device_info(int *device_id, int *n_cus, uint64_t *total_mem);
+
+/* ---- gate matrices -------------------------------------------------------------------- */
+
+/* Factory id for a gate type name (U, U2, U1, ID, X, Y, Z, H, S, T, RX, RY, RZ, ExpiI, ExpiZ,
+ * SH); replaces register_matrix_factory (glue.cpp:92-134). */
+int qghip_register_matrix_factory(const char *gate_type, int *factory_id);
+/* FP64 2x2 matrix of a gate, conjugate-transposed when `adjoint` != 0, as row-major
+ * (m00, m01, m10, m11) (re, im) pairs; the matrix built inside qubit_processor_apply_gate
+ * (glue.cpp:390-398, GateMatrix.cpp:13-158). */
+int qghip_gate_matrix(int factory_id, const double *args, int n_args, int adjoint, double *mat8);
+
+/* ---- qubit states --------------------------------------------------------------------- */
+
+/* qubit_states_new (cpuext.cpp:11-27, cudaext.cpp:70-88). */
+int qghip_qubit_states_new(int prec, qghip_qubit_states **out);
+/* qubit_states_delete (glue.cpp:193-205); frees the device memory. */
+int qghip_qubit_states_delete(qghip_qubit_states *qs);
+/* qubit_states_deallocate (glue.cpp:180-190). */
+int qghip_qubit_states_deallocate(qghip_qubit_states *qs);
+/* qubit_states_get_n_lanes (glue.cpp:237-246). */
+int qghip_qubit_states_get_n_lanes(const qghip_qubit_states *qs, int *n_lanes);
+
+/* ---- qubit processor (Interfaces.h:22-50) ------------------------------------------- */
+
+/* qubit_processor_new (cpuext.cpp:31-49, cudaext.cpp:90-107). */
+int qghip_qubit_processor_new(int prec, qghip_qubit_processor **out);
+/* qubit_processor_delete (glue.cpp:208-219). */
+int qghip_qubit_processor_delete(qghip_qubit_processor *p);
+/* qubit_processor_synchronize (glue.cpp:251-260): completes all queued device work. */
+int qghip_qubit_processor_synchronize(qghip_qubit_processor *p);
+/* qubit_processor_reset (glue.cpp:263-272). */
+int qghip_qubit_processor_reset(qghip_qubit_processor *p);
+/* qubit_processor_initialize_qubit_states (glue.cpp:275-286): allocate 2^n_lanes amplitudes. */
+int qghip_qubit_processor_initialize_qubit_states(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                  int n_lanes);
+/* qubit_processor_reset_qubit_states (glue.cpp:289-299): set |0...0>. */
+int qghip_qubit_processor_reset_qubit_states(qghip_qubit_processor *p, qghip_qubit_states *qs);
+/* qubit_processor_calc_probability (glue.cpp:302-311): P(bit local_lane == 0), FP64. */
+int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                           int local_lane, double *prob);
+/* qubit_processor_join (glue.cpp:314-334): qs <- kron(list[0], ..., list[n-1]) with the last
+ * entry in the lowest bits; the n_new_lanes top lanes start in |0>. */
+int qghip_qubit_processor_join(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                               qghip_qubit_states *const *qstates_list, int n_qstates,
+                               int n_new_lanes);
+/* qubit_processor_decohere (glue.cpp:337-348): collapse local_lane to `value` in place. */
+int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double prob,
+                                   qghip_qubit_states *qs, int local_lane);
+/* qubit_processor_decohere_and_separate (glue.cpp:351-365): collapse local_lane of qs to
+ * `value`; qs0 receives the other lanes (n-1), qs1 the measured lane. */
+int qghip_qubit_processor_decohere_and_separate(qghip_qubit_processor *p, int value, double prob,
+                                                qghip_qubit_states *qs0, qghip_qubit_states *qs1,
+                                                qghip_qubit_states *qs, int local_lane);
+/* qubit_processor_apply_reset (glue.cpp:368-379): move the |1> half of local_lane to |0>. */
+int qghip_qubit_processor_apply_reset(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                      int local_lane);
+/* qubit_processor_apply_gate (glue.cpp:383-405): matrix from factory + args (+ adjoint). */
+int qghip_qubit_processor_apply_gate(qghip_qubit_processor *p, int factory_id, const double *args,
+                                     int n_args, int adjoint, qghip_qubit_states *qs,
+                                     int local_lane);
+/* qubit_processor_apply_controlled_gate (glue.cpp:409-431). */
+int qghip_qubit_processor_apply_controlled_gate(qghip_qubit_processor *p, int factory_id,
+                                                const double *args, int n_args, int adjoint,
+                                                qghip_qubit_states *qs,
+                                                const int *local_control_lanes, int n_controls,
+                                                int local_target_lane);
+/* Same as apply_controlled_gate with an explicit FP64 matrix (n_controls may be 0);
+ * QubitProcessor::applyControlledGate (Interfaces.h:48-49). */
+int qghip_qubit_processor_apply_matrix(qghip_qubit_processor *p, const double *mat8,
+                                       qghip_qubit_states *qs, const int *local_control_lanes,
+                                       int n_controls, int local_target_lane);
+/* Addition, no reference counterpart: apply a dense 2^k x 2^k unitary (row-major (re, im)
+ * pairs, FP64, 1 <= k <= 6) to lanes[0..k-1]; lanes[j] is bit j of the matrix index. */
+int qghip_qubit_processor_apply_unitary(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                        const int *lanes, int k, const double *mat);
+
+/* ---- states getter (Interfaces.h:53-72) ---------------------------------------------- */
+
+/* qubits_states_getter_new (cpuext.cpp:52-70, cudaext.cpp:109-125). */
+int qghip_qubits_states_getter_new(int prec, qghip_qubits_states_getter **out);
+/* qubits_states_getter_delete (glue.cpp:222-233). */
+int qghip_qubits_states_getter_delete(qghip_qubits_states_getter *g);
+/* qubits_states_getter_get_states (glue.cpp:448-506): for j < n_states, ext = start + j*step,
+ * array[array_offset + j] = 0 if ext & empty_lane_mask, else the product over the state
+ * vectors of f(amplitude at the local index of ext); f = identity (QGHIP_MATHOP_NULL, complex
+ * output) or |.|^2 (QGHIP_MATHOP_PROB, real output) in the state precision.
+ * lane_tables concatenates, for each state vector in order, its local->external lane table.
+ * With n_qstates == 0 the output is 1 at index 0 and 0 elsewhere. Host output buffer. */
+int qghip_qubits_states_getter_get_states(qghip_qubits_states_getter *g, void *array,
+                                          int64_t array_size, int64_t array_offset, int mathop,
+                                          const int *lane_tables,
+                                          qghip_qubit_states *const *qstates_list, int n_qstates,
+                                          uint64_t empty_lane_mask, int n_qregs,
+                                          int64_t n_states, int64_t start, int64_t step);
+/* qubits_states_getter_prepare_prob_array (glue.cpp:509-536): prob[x] = sum over hidden
+ * lanes of the probability of external index x; external lanes >= n_lanes are hidden
+ * (placed in the most significant bits). Output in the state precision, host buffer. */
+int qghip_qubits_states_getter_prepare_prob_array(qghip_qubits_states_getter *g, void *prob,
+                                                  int64_t array_size, const int *lane_tables,
+                                                  qghip_qubit_states *const *qstates_list,
+                                                  int n_qstates, int n_lanes, int n_hidden_lanes);
+/* qubits_states_getter_create_sampling_pool (glue.cpp:539-559): device-resident FP64 CDF of
+ * the same marginal distribution; empty_lanes are external bit positions that read as 0. */
+int qghip_qubits_states_getter_create_sampling_pool(qghip_qubits_states_getter *g,
+                                                    const int *lane_tables,
+                                                    qghip_qubit_states *const *qstates_list,
+                                                    int n_qstates, int n_lanes,
+                                                    int n_hidden_lanes, const int *empty_lanes,
+                                                    int n_empty_lanes,
+                                                    qghip_sampling_pool **out);
+
+/* ---- sampling pool (Interfaces.h:75-81) ---------------------------------------------- */
+
+/* sampling_pool_sample (glue.cpp:562-589): observations[i] = first bin whose normalised
+ * cumulative probability exceeds randnum[i] (upper_bound), with zero bits inserted at the
+ * empty lanes. A randnum at or above the normalised total maps to the last nonzero bin. */
+int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations,
+                               int64_t array_size, int64_t n_samples, const double *randnum);
+/* sampling_pool_delete (glue.cpp:592-602). */
+int qghip_sampling_pool_delete(qghip_sampling_pool *pool);
+
+/* ---- measurement helper (no reference counterpart) ------------------------------------ */
+
+/* Time `reps` applications of one (controlled) gate with HIP events on the library stream;
+ * avg_ms receives the mean kernel duration. For roofline reporting. */
+int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const double *mat8,
+                       const int *local_control_lanes, int n_controls, int local_target_lane,
+                       int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGATE_HIP_H */

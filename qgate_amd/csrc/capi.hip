@@ -1,0 +1,571 @@
+// C ABI of the qgate HIP runtime (include/qgate_hip.h) and the objects behind the handles.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qgate_hip.h"
+#include "gate_matrix.hpp"
+#include "kernels.hpp"
+#include "qg_runtime.hpp"
+
+using namespace qg;
+
+struct qghip_qubit_states {
+    Prec prec;
+    int n_lanes = -1;
+    void *d_ptr = nullptr;
+    size_t bytes = 0;
+
+    size_t n_states() const { return size_t(1) << n_lanes; }
+};
+
+struct qghip_qubit_processor {
+    Prec prec;
+};
+
+struct qghip_qubits_states_getter {
+    Prec prec;
+};
+
+struct qghip_sampling_pool {
+    PoolData pool;
+    std::vector<int> empty_lanes;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ValueError : Error {
+    using Error::Error;
+};
+
+[[noreturn]] void value_error(const char *msg) { throw ValueError(msg); }
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        std::lock_guard<std::recursive_mutex> lk(runtime().mtx);
+        f();
+        return QGHIP_OK;
+    } catch (const ValueError &e) {
+        g_last_error = e.what();
+        return QGHIP_EVALUE;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return QGHIP_ERROR;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return QGHIP_ERROR;
+    }
+}
+
+Prec to_prec(int prec) {
+    if (prec == QGHIP_FP64) return precFP64;
+    if (prec == QGHIP_FP32) return precFP32;
+    value_error("precision must be QGHIP_FP64 or QGHIP_FP32");
+}
+
+void check_qs(const qghip_qubit_states *qs, Prec prec) {
+    if (qs == nullptr) value_error("null qubit states handle");
+    if (qs->d_ptr == nullptr) value_error("qubit states are not allocated");
+    if (qs->prec != prec) value_error("qubit states precision does not match");
+}
+
+void check_lane(const qghip_qubit_states *qs, int lane) {
+    if (lane < 0 || lane >= qs->n_lanes) value_error("lane out of range");
+}
+
+void release(qghip_qubit_states *qs) {
+    if (qs->d_ptr) dev_free(qs->d_ptr, qs->bytes);
+    qs->d_ptr = nullptr;
+    qs->bytes = 0;
+}
+
+template <class R>
+Cx<R> *amps(qghip_qubit_states *qs) {
+    return static_cast<Cx<R> *>(qs->d_ptr);
+}
+
+template <class R>
+const Cx<R> *camps(const qghip_qubit_states *qs) {
+    return static_cast<const Cx<R> *>(qs->d_ptr);
+}
+
+// Dispatch on precision: f(float()) or f(double()).
+template <class F>
+void by_prec(Prec p, F &&f) {
+    if (p == precFP32)
+        f(float());
+    else
+        f(double());
+}
+
+void apply_matrix(Prec prec, qghip_qubit_states *qs, const GateMatrix &m, const int *ctrl,
+                  int n_ctrl, int target) {
+    check_qs(qs, prec);
+    check_lane(qs, target);
+    if (n_ctrl < 0 || (n_ctrl > 0 && ctrl == nullptr)) value_error("bad control lane list");
+    std::vector<int> controls(ctrl, ctrl + n_ctrl);
+    for (int c : controls) {
+        check_lane(qs, c);
+        if (c == target) value_error("control and target lanes overlap");
+    }
+    std::vector<int> sorted(controls);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        value_error("duplicate control lanes");
+    by_prec(prec, [&](auto r) {
+        typedef decltype(r) R;
+        launch_gate<R>(amps<R>(qs), qs->n_lanes, m, controls, target);
+    });
+}
+
+std::vector<StateView> make_views(const int *lane_tables, qghip_qubit_states *const *list, int n,
+                                  Prec prec) {
+    if (n < 0 || (n > 0 && (list == nullptr || lane_tables == nullptr)))
+        value_error("bad qubit states list");
+    std::vector<StateView> views;
+    int off = 0;
+    for (int q = 0; q < n; ++q) {
+        check_qs(list[q], prec);
+        StateView v;
+        v.ptr = list[q]->d_ptr;
+        v.n_lanes = list[q]->n_lanes;
+        v.local_to_ext.assign(lane_tables + off, lane_tables + off + v.n_lanes);
+        off += v.n_lanes;
+        views.push_back(v);
+    }
+    return views;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qghip_last_error(void) { return g_last_error.c_str(); }
+
+int qghip_devices_initialize(int device_id) {
+    return guarded([&] { runtime_init(device_id); });
+}
+
+int qghip_devices_clear(void) {
+    return guarded([&] { runtime_clear(); });
+}
+
+int qghip_device_info(int *device_id, int *n_cus, uint64_t *total_mem) {
+    return guarded([&] {
+        stream();
+        Runtime &rt = runtime();
+        if (device_id) *device_id = rt.device;
+        if (n_cus) *n_cus = rt.n_cus;
+        if (total_mem) *total_mem = rt.total_mem;
+    });
+}
+
+int qghip_register_matrix_factory(const char *gate_type, int *factory_id) {
+    return guarded([&] {
+        if (gate_type == nullptr || factory_id == nullptr) value_error("null argument");
+        int id = matrix_factory_id(gate_type);
+        if (id < 0) raise_error("Unknown gate type, %s.", gate_type);
+        *factory_id = id;
+    });
+}
+
+int qghip_gate_matrix(int factory_id, const double *args, int n_args, int adjoint, double *mat8) {
+    return guarded([&] {
+        if (mat8 == nullptr || (n_args > 0 && args == nullptr)) value_error("null argument");
+        GateMatrix m = make_gate_matrix(factory_id, args, n_args, adjoint != 0);
+        gate_matrix_to_array(m, mat8);
+    });
+}
+
+int qghip_qubit_states_new(int prec, qghip_qubit_states **out) {
+    return guarded([&] {
+        if (out == nullptr) value_error("null argument");
+        qghip_qubit_states *qs = new qghip_qubit_states;
+        qs->prec = to_prec(prec);
+        *out = qs;
+    });
+}
+
+int qghip_qubit_states_delete(qghip_qubit_states *qs) {
+    return guarded([&] {
+        if (qs == nullptr) return;
+        release(qs);
+        delete qs;
+    });
+}
+
+int qghip_qubit_states_deallocate(qghip_qubit_states *qs) {
+    return guarded([&] {
+        if (qs == nullptr) value_error("null qubit states handle");
+        release(qs);
+    });
+}
+
+int qghip_qubit_states_get_n_lanes(const qghip_qubit_states *qs, int *n_lanes) {
+    return guarded([&] {
+        if (qs == nullptr || n_lanes == nullptr) value_error("null argument");
+        *n_lanes = qs->n_lanes;
+    });
+}
+
+int qghip_qubit_processor_new(int prec, qghip_qubit_processor **out) {
+    return guarded([&] {
+        if (out == nullptr) value_error("null argument");
+        qghip_qubit_processor *p = new qghip_qubit_processor;
+        p->prec = to_prec(prec);
+        *out = p;
+    });
+}
+
+int qghip_qubit_processor_delete(qghip_qubit_processor *p) {
+    return guarded([&] { delete p; });
+}
+
+int qghip_qubit_processor_synchronize(qghip_qubit_processor *p) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        sync();
+    });
+}
+
+int qghip_qubit_processor_reset(qghip_qubit_processor *p) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+    });
+}
+
+int qghip_qubit_processor_initialize_qubit_states(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                  int n_lanes) {
+    return guarded([&] {
+        if (p == nullptr || qs == nullptr) value_error("null argument");
+        if (qs->prec != p->prec) value_error("qubit states precision does not match");
+        if (n_lanes < 1 || n_lanes > 40) value_error("n_lanes out of range (1..40)");
+        release(qs);
+        qs->n_lanes = n_lanes;
+        qs->bytes = amp_bytes(qs->prec) << n_lanes;
+        qs->d_ptr = dev_alloc(qs->bytes);
+    });
+}
+
+int qghip_qubit_processor_reset_qubit_states(qghip_qubit_processor *p, qghip_qubit_states *qs) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_set_zero_state<R>(amps<R>(qs), qs->n_lanes);
+        });
+    });
+}
+
+int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                           int local_lane, double *prob) {
+    return guarded([&] {
+        if (p == nullptr || prob == nullptr) value_error("null argument");
+        check_qs(qs, p->prec);
+        check_lane(qs, local_lane);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            *prob = calc_prob<R>(camps<R>(qs), qs->n_lanes, local_lane);
+        });
+    });
+}
+
+int qghip_qubit_processor_join(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                               qghip_qubit_states *const *qstates_list, int n_qstates,
+                               int n_new_lanes) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        if (n_qstates < 1 || qstates_list == nullptr) value_error("empty qubit states list");
+        int total = 0;
+        std::vector<int> lanes;
+        for (int i = 0; i < n_qstates; ++i) {
+            check_qs(qstates_list[i], p->prec);
+            if (qstates_list[i] == qs) value_error("join destination is also a source");
+            lanes.push_back(qstates_list[i]->n_lanes);
+            total += qstates_list[i]->n_lanes;
+        }
+        if (total + n_new_lanes != qs->n_lanes) value_error("joined lane count mismatch");
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            std::vector<const Cx<R> *> srcs;
+            for (int i = 0; i < n_qstates; ++i) srcs.push_back(camps<R>(qstates_list[i]));
+            launch_join<R>(amps<R>(qs), qs->n_lanes, srcs, lanes);
+        });
+    });
+}
+
+int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double prob,
+                                   qghip_qubit_states *qs, int local_lane) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        check_lane(qs, local_lane);
+        if (value != 0 && value != 1) value_error("value must be 0 or 1");
+        // norm in FP64, rounded to the state precision (CPUQubitProcessor.cpp:227, :235)
+        const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_decohere<R>(amps<R>(qs), qs->n_lanes, local_lane, value, (R)norm);
+        });
+    });
+}
+
+int qghip_qubit_processor_decohere_and_separate(qghip_qubit_processor *p, int value, double prob,
+                                                qghip_qubit_states *qs0, qghip_qubit_states *qs1,
+                                                qghip_qubit_states *qs, int local_lane) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        check_qs(qs0, p->prec);
+        check_qs(qs1, p->prec);
+        check_lane(qs, local_lane);
+        if (value != 0 && value != 1) value_error("value must be 0 or 1");
+        if (qs0->n_lanes != qs->n_lanes - 1 || qs1->n_lanes != 1)
+            value_error("separated qubit states have wrong sizes");
+        const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_decohere_and_separate<R>(amps<R>(qs0), camps<R>(qs), qs->n_lanes, local_lane,
+                                            value, (R)norm);
+            launch_set_basis1<R>(amps<R>(qs1), value);
+        });
+    });
+}
+
+int qghip_qubit_processor_apply_reset(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                      int local_lane) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        check_lane(qs, local_lane);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_reset<R>(amps<R>(qs), qs->n_lanes, local_lane);
+        });
+    });
+}
+
+int qghip_qubit_processor_apply_gate(qghip_qubit_processor *p, int factory_id, const double *args,
+                                     int n_args, int adjoint, qghip_qubit_states *qs,
+                                     int local_lane) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        GateMatrix m = make_gate_matrix(factory_id, args, n_args, adjoint != 0);
+        apply_matrix(p->prec, qs, m, nullptr, 0, local_lane);
+    });
+}
+
+int qghip_qubit_processor_apply_controlled_gate(qghip_qubit_processor *p, int factory_id,
+                                                const double *args, int n_args, int adjoint,
+                                                qghip_qubit_states *qs,
+                                                const int *local_control_lanes, int n_controls,
+                                                int local_target_lane) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        GateMatrix m = make_gate_matrix(factory_id, args, n_args, adjoint != 0);
+        apply_matrix(p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
+    });
+}
+
+int qghip_qubit_processor_apply_matrix(qghip_qubit_processor *p, const double *mat8,
+                                       qghip_qubit_states *qs, const int *local_control_lanes,
+                                       int n_controls, int local_target_lane) {
+    return guarded([&] {
+        if (p == nullptr || mat8 == nullptr) value_error("null argument");
+        apply_matrix(p->prec, qs, gate_matrix_from_array(mat8), local_control_lanes, n_controls,
+                     local_target_lane);
+    });
+}
+
+int qghip_qubit_processor_apply_unitary(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                        const int *lanes, int k, const double *mat) {
+    return guarded([&] {
+        if (p == nullptr || lanes == nullptr || mat == nullptr) value_error("null argument");
+        check_qs(qs, p->prec);
+        if (k < 1 || k > 6 || k > qs->n_lanes) value_error("unitary width out of range");
+        std::vector<int> ls(lanes, lanes + k);
+        for (int l : ls) check_lane(qs, l);
+        const int D = 1 << k;
+        std::vector<std::complex<double>> m(D * D);
+        for (int i = 0; i < D * D; ++i) m[i] = std::complex<double>(mat[2 * i], mat[2 * i + 1]);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_unitary<R>(amps<R>(qs), qs->n_lanes, ls, m);
+        });
+    });
+}
+
+int qghip_qubits_states_getter_new(int prec, qghip_qubits_states_getter **out) {
+    return guarded([&] {
+        if (out == nullptr) value_error("null argument");
+        qghip_qubits_states_getter *g = new qghip_qubits_states_getter;
+        g->prec = to_prec(prec);
+        *out = g;
+    });
+}
+
+int qghip_qubits_states_getter_delete(qghip_qubits_states_getter *g) {
+    return guarded([&] { delete g; });
+}
+
+int qghip_qubits_states_getter_get_states(qghip_qubits_states_getter *g, void *array,
+                                          int64_t array_size, int64_t array_offset, int mathop,
+                                          const int *lane_tables,
+                                          qghip_qubit_states *const *qstates_list, int n_qstates,
+                                          uint64_t empty_lane_mask, int n_qregs,
+                                          int64_t n_states, int64_t start, int64_t step) {
+    return guarded([&] {
+        if (g == nullptr || array == nullptr) value_error("null argument");
+        if (mathop != QGHIP_MATHOP_NULL && mathop != QGHIP_MATHOP_PROB) value_error("Unknown math op.");
+        if (n_states < 0 || array_offset < 0 || array_size - array_offset < n_states)
+            value_error("array size too small.");
+        if (n_qregs < 0 || n_qregs > 62) value_error("n_qregs out of range");
+        if (n_states == 0) return;
+        const int64_t n_ext = int64_t(1) << n_qregs;
+        const int64_t end = start + step * (n_states - 1);
+        if (start < 0 || n_ext <= start || end < 0 || n_ext <= end) value_error("value out of range");
+        const size_t real_size = g->prec == precFP64 ? 8 : 4;
+        const size_t item = mathop == QGHIP_MATHOP_NULL ? 2 * real_size : real_size;
+        char *out = static_cast<char *>(array) + item * (size_t)array_offset;
+        if (n_qstates == 0) {
+            // no state vector: |0...0> (glue.cpp:480-495)
+            std::memset(out, 0, item * (size_t)n_states);
+            if (start == 0) {
+                if (real_size == 4) {
+                    const float one = 1.f;
+                    std::memcpy(out, &one, 4);
+                } else {
+                    const double one = 1.;
+                    std::memcpy(out, &one, 8);
+                }
+            }
+            return;
+        }
+        std::vector<StateView> views = make_views(lane_tables, qstates_list, n_qstates, g->prec);
+        by_prec(g->prec, [&](auto r) {
+            typedef decltype(r) R;
+            get_states<R>(out, mathop, views, empty_lane_mask, n_states, start, step);
+        });
+    });
+}
+
+int qghip_qubits_states_getter_prepare_prob_array(qghip_qubits_states_getter *g, void *prob,
+                                                  int64_t array_size, const int *lane_tables,
+                                                  qghip_qubit_states *const *qstates_list,
+                                                  int n_qstates, int n_lanes, int n_hidden_lanes) {
+    return guarded([&] {
+        if (g == nullptr || prob == nullptr) value_error("null argument");
+        if (n_lanes < 0 || n_hidden_lanes < 0 || n_lanes + n_hidden_lanes > 62)
+            value_error("lane counts out of range");
+        const int64_t n_states = int64_t(1) << n_lanes;
+        if (array_size < n_states) value_error("array size too small.");
+        std::vector<StateView> views = make_views(lane_tables, qstates_list, n_qstates, g->prec);
+        by_prec(g->prec, [&](auto r) {
+            typedef decltype(r) R;
+            const size_t bytes = sizeof(R) * (size_t)n_states;
+            void *d = dev_alloc(bytes);
+            try {
+                launch_prob_array<R>(d, false, views, n_lanes, n_hidden_lanes);
+                QG_HIP(hipMemcpyAsync(prob, d, bytes, hipMemcpyDeviceToHost, stream()));
+                sync();
+            } catch (...) {
+                dev_free(d, bytes);
+                throw;
+            }
+            dev_free(d, bytes);
+        });
+    });
+}
+
+int qghip_qubits_states_getter_create_sampling_pool(qghip_qubits_states_getter *g,
+                                                    const int *lane_tables,
+                                                    qghip_qubit_states *const *qstates_list,
+                                                    int n_qstates, int n_lanes,
+                                                    int n_hidden_lanes, const int *empty_lanes,
+                                                    int n_empty_lanes, qghip_sampling_pool **out) {
+    return guarded([&] {
+        if (g == nullptr || out == nullptr) value_error("null argument");
+        if (n_lanes < 1 || n_hidden_lanes < 0 || n_lanes + n_hidden_lanes > 62)
+            value_error("lane counts out of range");
+        if (n_empty_lanes < 0 || (n_empty_lanes > 0 && empty_lanes == nullptr))
+            value_error("bad empty lane list");
+        std::vector<StateView> views = make_views(lane_tables, qstates_list, n_qstates, g->prec);
+        qghip_sampling_pool *pool = new qghip_sampling_pool;
+        pool->empty_lanes.assign(empty_lanes, empty_lanes + n_empty_lanes);
+        try {
+            bool direct = n_hidden_lanes == 0 && views.size() == 1 && views[0].n_lanes == n_lanes;
+            for (int l = 0; direct && l < n_lanes; ++l) direct = views[0].local_to_ext[l] == l;
+            by_prec(g->prec, [&](auto r) {
+                typedef decltype(r) R;
+                if (direct) {
+                    pool_build_from_state<R>(pool->pool, static_cast<const Cx<R> *>(views[0].ptr), n_lanes);
+                } else {
+                    pool_alloc(pool->pool, n_lanes);
+                    try {
+                        launch_prob_array<R>(pool->pool.p, true, views, n_lanes, n_hidden_lanes);
+                        pool_build_from_probs(pool->pool, n_lanes);
+                    } catch (...) {
+                        pool_free(pool->pool);
+                        throw;
+                    }
+                }
+            });
+        } catch (...) {
+            delete pool;
+            throw;
+        }
+        *out = pool;
+    });
+}
+
+int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations, int64_t array_size,
+                               int64_t n_samples, const double *randnum) {
+    return guarded([&] {
+        if (pool == nullptr || observations == nullptr || randnum == nullptr)
+            value_error("null argument");
+        if (n_samples < 0 || array_size < n_samples) value_error("array size too small.");
+        pool_sample(pool->pool, reinterpret_cast<i64 *>(observations), n_samples, randnum,
+                    pool->empty_lanes);
+    });
+}
+
+int qghip_sampling_pool_delete(qghip_sampling_pool *pool) {
+    return guarded([&] {
+        if (pool == nullptr) return;
+        pool_free(pool->pool);
+        delete pool;
+    });
+}
+
+int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const double *mat8,
+                       const int *local_control_lanes, int n_controls, int local_target_lane,
+                       int reps, double *avg_ms) {
+    return guarded([&] {
+        if (p == nullptr || mat8 == nullptr || avg_ms == nullptr || reps < 1)
+            value_error("bad argument");
+        GateMatrix m = gate_matrix_from_array(mat8);
+        hipStream_t s = stream();
+        hipEvent_t e0, e1;
+        QG_HIP(hipEventCreate(&e0));
+        QG_HIP(hipEventCreate(&e1));
+        QG_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i)
+            apply_matrix(p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
+        QG_HIP(hipEventRecord(e1, s));
+        QG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        *avg_ms = ms / reps;
+    });
+}
+
+}  // extern "C"

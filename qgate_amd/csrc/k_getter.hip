@@ -1,0 +1,448 @@
+// Readout kernels: amplitudes / probabilities in the external qreg ordering, marginal
+// probability arrays, and the FP64 sampling pool (prefix sum + batched search).
+//
+// Reference semantics:
+//   get_states        qgate/simulator/src/CPUQubitsStatesGetter.cpp:34-157, glue.cpp:447-506
+//   prepareProbArray  CPUQubitsStatesGetter.cpp:159-254, DeviceProbArrayCalculator.cu
+//   SamplingPool      CPUSamplingPool.cpp:8-81: inclusive prefix sum, normalised by 1/sum,
+//                     observation = upper_bound(cdf, u), then zero bits inserted at the
+//                     empty-lane positions.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+#include "qg_runtime.hpp"
+
+namespace qg {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxQs = 64;
+
+inline unsigned blocks_for(u64 n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+struct ViewArgs {
+    const void *ptr[kMaxQs];
+    int lane_begin[kMaxQs + 1];
+    int ext_pos[64];   // external bit of each lane, lanes of qstates q at [lane_begin[q], lane_begin[q+1])
+    int n_qs;
+};
+
+__device__ __forceinline__ u64 local_index(u64 ext, const ViewArgs &g, int q) {
+    u64 local = 0;
+    const int b = g.lane_begin[q], e = g.lane_begin[q + 1];
+    for (int l = b; l < e; ++l) local |= ((ext >> g.ext_pos[l]) & 1) << (l - b);
+    return local;
+}
+
+template <class R>
+__device__ __forceinline__ R abs2r(const Cx<R> &c) {
+    return c.re * c.re + c.im * c.im;
+}
+
+// mathop 0: product of amplitudes; mathop 1: product of |amplitude|^2 (in R, as the
+// reference getter does).
+template <class R, int MATHOP>
+__global__ __launch_bounds__(kBlock) void k_get_states(void *out, const ViewArgs g, u64 empty_mask,
+                                                       i64 start, i64 step, u64 count) {
+    const u64 j = (u64)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= count) return;
+    const u64 ext = (u64)(start + (i64)j * step);
+    if (MATHOP == 0) {
+        Cx<R> v = cx<R>(1, 0);
+        if (ext & empty_mask) {
+            v = cx<R>(0, 0);
+        } else {
+            for (int q = 0; q < g.n_qs; ++q)
+                v = cmul(v, static_cast<const Cx<R> *>(g.ptr[q])[local_index(ext, g, q)]);
+        }
+        static_cast<Cx<R> *>(out)[j] = v;
+    } else {
+        R v = R(1);
+        if (ext & empty_mask) {
+            v = R(0);
+        } else {
+            for (int q = 0; q < g.n_qs; ++q)
+                v *= abs2r(static_cast<const Cx<R> *>(g.ptr[q])[local_index(ext, g, q)]);
+        }
+        static_cast<R *>(out)[j] = v;
+    }
+}
+
+// out[x] = sum_h prod_q |a_q(x | h << n_lanes)|^2, FP64 accumulation.
+template <class R, class O>
+__global__ __launch_bounds__(kBlock) void k_prob_array(O *out, const ViewArgs g, int n_lanes,
+                                                       u64 n_hidden_states, u64 n_out) {
+    const u64 x = (u64)blockIdx.x * kBlock + threadIdx.x;
+    if (x >= n_out) return;
+    double s = 0.;
+    for (u64 h = 0; h < n_hidden_states; ++h) {
+        const u64 ext = x | (h << n_lanes);
+        double v = 1.;
+        for (int q = 0; q < g.n_qs; ++q)
+            v *= abs2d(static_cast<const Cx<R> *>(g.ptr[q])[local_index(ext, g, q)]);
+        s += v;
+    }
+    out[x] = (O)s;
+}
+
+ViewArgs make_view(const std::vector<StateView> &qs) {
+    QG_REQUIRE((int)qs.size() <= kMaxQs, "too many state vectors (%zu)", qs.size());
+    ViewArgs g;
+    g.n_qs = (int)qs.size();
+    int l = 0;
+    for (int q = 0; q < g.n_qs; ++q) {
+        g.ptr[q] = qs[q].ptr;
+        g.lane_begin[q] = l;
+        QG_REQUIRE((int)qs[q].local_to_ext.size() == qs[q].n_lanes, "lane table size mismatch");
+        for (int e : qs[q].local_to_ext) {
+            QG_REQUIRE(l < 64, "more than 63 lanes");
+            QG_REQUIRE(0 <= e && e < 63, "external lane %d out of range", e);
+            g.ext_pos[l++] = e;
+        }
+    }
+    g.lane_begin[g.n_qs] = l;
+    return g;
+}
+
+bool is_identity_view(const std::vector<StateView> &qs) {
+    if (qs.size() != 1) return false;
+    for (int l = 0; l < qs[0].n_lanes; ++l)
+        if (qs[0].local_to_ext[l] != l) return false;
+    return true;
+}
+
+}  // namespace
+
+template <class R>
+void get_states(void *host_out, int mathop, const std::vector<StateView> &qs, u64 empty_mask,
+                i64 n_states, i64 start, i64 step) {
+    QG_REQUIRE(mathop == 0 || mathop == 1, "Unknown math op %d.", mathop);
+    if (n_states <= 0) return;
+    hipStream_t s = stream();
+    const size_t item = mathop == 0 ? sizeof(Cx<R>) : sizeof(R);
+    if (mathop == 0 && step == 1 && empty_mask == 0 && is_identity_view(qs)) {
+        // external order == local order: one device-to-host copy
+        const Cx<R> *a = static_cast<const Cx<R> *>(qs[0].ptr);
+        QG_HIP(hipMemcpyAsync(host_out, a + start, item * (size_t)n_states, hipMemcpyDeviceToHost, s));
+        QG_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    const ViewArgs g = make_view(qs);
+    const i64 chunk = i64(1) << 26;
+    const i64 staged = std::min<i64>(chunk, n_states);
+    void *d_out = dev_alloc(item * (size_t)staged);
+    try {
+        for (i64 done = 0; done < n_states; done += chunk) {
+            const i64 cnt = std::min<i64>(chunk, n_states - done);
+            const i64 st = start + done * step;
+            if (mathop == 0)
+                hipLaunchKernelGGL((k_get_states<R, 0>), blocks_for(cnt), kBlock, 0, s, d_out, g,
+                                   empty_mask, st, step, (u64)cnt);
+            else
+                hipLaunchKernelGGL((k_get_states<R, 1>), blocks_for(cnt), kBlock, 0, s, d_out, g,
+                                   empty_mask, st, step, (u64)cnt);
+            QG_HIP(hipGetLastError());
+            QG_HIP(hipMemcpyAsync(static_cast<char *>(host_out) + item * done, d_out, item * cnt,
+                                  hipMemcpyDeviceToHost, s));
+            QG_HIP(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        dev_free(d_out, item * (size_t)staged);
+        throw;
+    }
+    dev_free(d_out, item * (size_t)staged);
+}
+
+template <class R>
+void launch_prob_array(void *d_out, bool out_double, const std::vector<StateView> &qs, int n_lanes,
+                       int n_hidden) {
+    const ViewArgs g = make_view(qs);
+    const u64 n_out = u64(1) << n_lanes;
+    const u64 n_h = u64(1) << n_hidden;
+    hipStream_t s = stream();
+    if (out_double)
+        hipLaunchKernelGGL((k_prob_array<R, double>), blocks_for(n_out), kBlock, 0, s,
+                           static_cast<double *>(d_out), g, n_lanes, n_h, n_out);
+    else
+        hipLaunchKernelGGL((k_prob_array<R, R>), blocks_for(n_out), kBlock, 0, s,
+                           static_cast<R *>(d_out), g, n_lanes, n_h, n_out);
+    QG_HIP(hipGetLastError());
+}
+
+template void get_states<float>(void *, int, const std::vector<StateView> &, u64, i64, i64, i64);
+template void get_states<double>(void *, int, const std::vector<StateView> &, u64, i64, i64, i64);
+template void launch_prob_array<float>(void *, bool, const std::vector<StateView> &, int, int);
+template void launch_prob_array<double>(void *, bool, const std::vector<StateView> &, int, int);
+
+// ---- sampling pool -----------------------------------------------------------------------
+//
+// CDF definition (FP64, deterministic, monotone). Bins are grouped into chunks of 16,
+// tiles of 256 chunks and super-tiles of 1024 tiles. With P the running sum inside a chunk,
+// Es / Ms / Hs the inclusive prefix of the previous chunk in the tile / tile in the
+// super-tile / super-tile in the pool (0 for the first), each computed serially:
+//     cdf(bin) = Hs + (Ms + (Es + P))          (each + rounded in FP64)
+// The value at the end of a chunk, tile or super-tile equals the stored inclusive prefix,
+// so cdf never decreases across a boundary. A draw u returns the first bin with
+// cdf * (1 / total) > u (upper_bound semantics), found level by level.
+
+namespace {
+
+constexpr int kChunk = 16;
+constexpr int kTileChunks = 256;
+constexpr int kTile = kChunk * kTileChunks;   // 4096 bins
+constexpr int kSuperTiles = 1024;             // 2^22 bins per super-tile
+
+struct StateSrc {};
+struct ProbSrc {};
+
+// One block per tile: p = |a|^2 (from the state) or p given; chunk sums; serial scan of the
+// 256 chunk sums; tile total; last bin with p > 0.
+template <class R, bool FROM_STATE>
+__global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
+                                                      double *__restrict__ p, double *__restrict__ e_inc,
+                                                      double *__restrict__ t_tot, i64 *last_nonzero) {
+    __shared__ double sp[kTile + kTile / kChunk];   // one pad slot per chunk
+    __shared__ double sc[kTileChunks];
+    const i64 base = (i64)blockIdx.x * kTile;
+    for (int r = 0; r < kChunk; ++r) {
+        const int li = r * kBlock + threadIdx.x;
+        const i64 bin = base + li;
+        double v = 0.;
+        if (bin < n_bins) {
+            if (FROM_STATE) {
+                v = abs2d(a[bin]);
+                p[bin] = v;
+            } else {
+                v = p[bin];
+            }
+        } else {
+            p[bin] = 0.;   // padding up to whole tiles reads as zero probability
+        }
+        sp[li + li / kChunk] = v;
+    }
+    __syncthreads();
+    // serial sum over this thread's chunk
+    const int c = threadIdx.x;
+    double s = 0.;
+    i64 last = -1;
+    for (int k = 0; k < kChunk; ++k) {
+        const int li = c * kChunk + k;
+        const double v = sp[li + li / kChunk];
+        s = (k == 0) ? v : s + v;
+        if (v > 0.) last = base + li;
+    }
+    sc[c] = s;
+    // stored as last + 1 so that 0 means "no nonzero bin"
+    if (last >= 0) atomicMax(reinterpret_cast<unsigned long long *>(last_nonzero), (unsigned long long)(last + 1));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.;
+        for (int i = 0; i < kTileChunks; ++i) {
+            acc = (i == 0) ? sc[0] : acc + sc[i];
+            sc[i] = acc;
+        }
+        t_tot[blockIdx.x] = acc;
+    }
+    __syncthreads();
+    e_inc[(i64)blockIdx.x * kTileChunks + c] = sc[c];
+}
+
+// One wave per super-tile: serial inclusive scan of the tile totals in place; super total out.
+__global__ __launch_bounds__(64) void k_pool_super(double *__restrict__ m_inc, i64 n_tiles,
+                                                   double *__restrict__ s_tot) {
+    __shared__ double st[kSuperTiles];
+    const i64 base = (i64)blockIdx.x * kSuperTiles;
+    for (int i = threadIdx.x; i < kSuperTiles; i += 64)
+        st[i] = (base + i < n_tiles) ? m_inc[base + i] : 0.;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.;
+        for (int i = 0; i < kSuperTiles; ++i) {
+            acc = (i == 0) ? st[0] : acc + st[i];
+            st[i] = acc;
+        }
+        s_tot[blockIdx.x] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kSuperTiles; i += 64)
+        if (base + i < n_tiles) m_inc[base + i] = st[i];
+}
+
+// Serial inclusive scan of the super-tile totals in place; the last value is the total.
+__global__ void k_pool_top(double *h_inc, i64 n_supers, double *total) {
+    double acc = 0.;
+    for (i64 i = 0; i < n_supers; ++i) {
+        acc = (i == 0) ? h_inc[0] : acc + h_inc[i];
+        h_inc[i] = acc;
+    }
+    *total = acc;
+}
+
+struct SampleArgs {
+    const double *p, *e_inc, *m_inc, *h_inc;
+    i64 n_chunks, n_tiles, n_supers;
+    double norm;
+    i64 last_nonzero;
+    BitInsert empty;
+};
+
+// first index in [lo, hi) with f(index) true, for f monotone false..true; hi if none
+template <class F>
+__device__ __forceinline__ i64 first_true(i64 lo, i64 hi, F f) {
+    while (lo < hi) {
+        const i64 mid = lo + (hi - lo) / 2;
+        if (f(mid))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pool_sample(const SampleArgs g, const double *__restrict__ u,
+                                                        i64 *__restrict__ obs, i64 n) {
+    const i64 i = (i64)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double x = u[i];
+    const double norm = g.norm;
+    i64 bin;
+    const i64 gsup = first_true(0, g.n_supers, [&](i64 s) { return g.h_inc[s] * norm > x; });
+    if (gsup == g.n_supers) {
+        bin = g.last_nonzero;   // u beyond the normalised total: clamp to the last nonzero bin
+    } else {
+        const double hs = gsup > 0 ? g.h_inc[gsup - 1] : 0.;
+        const i64 t0 = gsup * kSuperTiles;
+        const i64 t1 = (t0 + kSuperTiles < g.n_tiles) ? t0 + kSuperTiles : g.n_tiles;
+        i64 t = first_true(t0, t1, [&](i64 k) { return (hs + g.m_inc[k]) * norm > x; });
+        if (t == t1) t = t1 - 1;
+        const double ms = t > t0 ? g.m_inc[t - 1] : 0.;
+        const i64 c0 = t * kTileChunks, c1 = c0 + kTileChunks;
+        i64 c = first_true(c0, c1, [&](i64 k) { return (hs + (ms + g.e_inc[k])) * norm > x; });
+        if (c == c1) c = c1 - 1;
+        const double es = c > c0 ? g.e_inc[c - 1] : 0.;
+        const double *pc = g.p + c * kChunk;
+        double run = 0.;
+        bin = c * kChunk + kChunk - 1;
+        for (int k = 0; k < kChunk; ++k) {
+            run = (k == 0) ? pc[0] : run + pc[k];
+            if ((hs + (ms + (es + run))) * norm > x) {
+                bin = c * kChunk + k;
+                break;
+            }
+        }
+    }
+    obs[i] = (i64)insert_zero_bits((u64)bin, g.empty);
+}
+
+}  // namespace
+
+void pool_alloc(PoolData &pd, int n_lanes) {
+    pd.n = i64(1) << n_lanes;
+    pd.n_tiles = (pd.n + kTile - 1) / kTile;
+    pd.n_chunks = pd.n_tiles * kTileChunks;
+    pd.n_supers = (pd.n_tiles + kSuperTiles - 1) / kSuperTiles;
+    const size_t n_p = (size_t)pd.n_tiles * kTile;
+    const size_t n_m = (size_t)pd.n_supers * kSuperTiles;
+    pd.bytes = sizeof(double) * (n_p + pd.n_chunks + n_m + pd.n_supers + 8);
+    pd.block = dev_alloc(pd.bytes);
+    double *d = static_cast<double *>(pd.block);
+    pd.p = d;
+    pd.e_inc = pd.p + n_p;
+    pd.m_inc = pd.e_inc + pd.n_chunks;
+    pd.h_inc = pd.m_inc + n_m;
+    pd.d_last_nonzero = reinterpret_cast<i64 *>(pd.h_inc + pd.n_supers);
+}
+
+void pool_free(PoolData &pd) {
+    if (pd.block) dev_free(pd.block, pd.bytes);
+    pd.block = nullptr;
+}
+
+namespace {
+
+template <class R, bool FROM_STATE>
+void pool_build(PoolData &pd, const Cx<R> *a) {
+    hipStream_t s = stream();
+    double *d_total = pd.h_inc + pd.n_supers + 1;
+    QG_HIP(hipMemsetAsync(pd.d_last_nonzero, 0, sizeof(i64), s));
+    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE>), (unsigned)pd.n_tiles, kBlock, 0, s, a, pd.n,
+                       pd.p, pd.e_inc, pd.m_inc, pd.d_last_nonzero);
+    QG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc);
+    QG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pool_top, 1, 1, 0, s, pd.h_inc, pd.n_supers, d_total);
+    QG_HIP(hipGetLastError());
+    Runtime &rt = runtime();
+    QG_HIP(hipMemcpyAsync(rt.h_pinned, d_total, sizeof(double), hipMemcpyDeviceToHost, s));
+    QG_HIP(hipMemcpyAsync(rt.h_pinned + 1, pd.d_last_nonzero, sizeof(i64), hipMemcpyDeviceToHost, s));
+    QG_HIP(hipStreamSynchronize(s));
+    pd.total = rt.h_pinned[0];
+    std::memcpy(&pd.last_nonzero, rt.h_pinned + 1, sizeof(i64));
+    pd.last_nonzero -= 1;
+    QG_REQUIRE(!(0.05 < std::abs(pd.total - 1.)),
+               "error in probability sum is beyond 0.05., %g.", pd.total);
+    pd.norm = 1. / pd.total;
+}
+
+}  // namespace
+
+template <class R>
+void pool_build_from_state(PoolData &pd, const Cx<R> *a, int n_lanes) {
+    pool_alloc(pd, n_lanes);
+    try {
+        pool_build<R, true>(pd, a);
+    } catch (...) {
+        pool_free(pd);
+        throw;
+    }
+}
+
+template void pool_build_from_state<float>(PoolData &, const Cx<float> *, int);
+template void pool_build_from_state<double>(PoolData &, const Cx<double> *, int);
+
+void pool_build_from_probs(PoolData &pd, int n_lanes) {
+    (void)n_lanes;
+    pool_build<double, false>(pd, nullptr);
+}
+
+void pool_sample(const PoolData &pd, i64 *host_obs, i64 n_samples, const double *host_u,
+                 const std::vector<int> &empty_lanes) {
+    if (n_samples <= 0) return;
+    SampleArgs g;
+    g.p = pd.p;
+    g.e_inc = pd.e_inc;
+    g.m_inc = pd.m_inc;
+    g.h_inc = pd.h_inc;
+    g.n_chunks = pd.n_chunks;
+    g.n_tiles = pd.n_tiles;
+    g.n_supers = pd.n_supers;
+    g.norm = pd.norm;
+    g.last_nonzero = pd.last_nonzero < 0 ? 0 : pd.last_nonzero;
+    std::vector<int> empty(empty_lanes);
+    std::sort(empty.begin(), empty.end());
+    QG_REQUIRE(empty.size() < 63, "too many empty lanes");
+    g.empty.n = (int)empty.size();
+    for (size_t i = 0; i < empty.size(); ++i) g.empty.pos[i] = empty[i];
+    hipStream_t s = stream();
+    const size_t bytes = sizeof(double) * (size_t)n_samples;
+    char *buf = static_cast<char *>(dev_alloc(2 * bytes));
+    double *d_u = reinterpret_cast<double *>(buf);
+    i64 *d_obs = reinterpret_cast<i64 *>(buf + bytes);
+    try {
+        QG_HIP(hipMemcpyAsync(d_u, host_u, bytes, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pool_sample, blocks_for((u64)n_samples), kBlock, 0, s, g, d_u, d_obs,
+                           n_samples);
+        QG_HIP(hipGetLastError());
+        QG_HIP(hipMemcpyAsync(host_obs, d_obs, bytes, hipMemcpyDeviceToHost, s));
+        QG_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        dev_free(buf, 2 * bytes);
+        throw;
+    }
+    dev_free(buf, 2 * bytes);
+}
+
+}  // namespace qg

@@ -1,0 +1,153 @@
+// Device context, stream and caching allocator.
+// Counterpart of the reference device setup and memory store
+// (qgate/simulator/src/CUDADevice.cpp:60-153, MultiDeviceMemoryStore.cpp:164-269), for one GPU:
+// no chunking across devices, and no device-wide synchronisation on allocate/free.
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <vector>
+
+#include "qg_runtime.hpp"
+
+namespace qg {
+
+void raise_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list va;
+    va_start(va, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, va);
+    va_end(va);
+    throw Error(buf);
+}
+
+Runtime &runtime() {
+    static Runtime rt;
+    return rt;
+}
+
+namespace {
+
+constexpr size_t kCacheMaxBlock = size_t(1) << 30;   // cache blocks up to 1 GiB
+constexpr size_t kCacheMaxTotal = size_t(8) << 30;   // keep at most 8 GiB cached
+
+std::map<size_t, std::vector<void *>> &free_lists() {
+    static std::map<size_t, std::vector<void *>> lists;
+    return lists;
+}
+size_t g_cached = 0;
+
+size_t size_class(size_t bytes) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+void release_cache() {
+    for (auto &kv : free_lists())
+        for (void *p : kv.second) hipFree(p);
+    free_lists().clear();
+    g_cached = 0;
+}
+
+}  // namespace
+
+void runtime_init(int device) {
+    Runtime &rt = runtime();
+    std::lock_guard<std::recursive_mutex> lk(rt.mtx);
+    if (rt.initialized) {
+        QG_REQUIRE(rt.device == device, "runtime already initialised on device %d", rt.device);
+        return;
+    }
+    int count = 0;
+    QG_HIP(hipGetDeviceCount(&count));
+    QG_REQUIRE(count > 0, "no HIP device is visible");
+    QG_REQUIRE(0 <= device && device < count, "device %d out of range (%d devices)", device, count);
+    QG_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    QG_HIP(hipGetDeviceProperties(&prop, device));
+    rt.device = device;
+    rt.n_cus = prop.multiProcessorCount;
+    rt.total_mem = prop.totalGlobalMem;
+    // default priority stream (non-blocking w.r.t. the legacy null stream)
+    QG_HIP(hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking));
+    QG_HIP(hipMalloc(&rt.d_partials, sizeof(double) * (reduce_grid() + 64)));
+    QG_HIP(hipHostMalloc(&rt.h_pinned, sizeof(double) * 64, hipHostMallocDefault));
+    rt.initialized = true;
+}
+
+void runtime_clear() {
+    Runtime &rt = runtime();
+    std::lock_guard<std::recursive_mutex> lk(rt.mtx);
+    if (!rt.initialized) return;
+    hipStreamSynchronize(rt.stream);
+    release_cache();
+    hipFree(rt.d_partials);
+    hipHostFree(rt.h_pinned);
+    hipStreamDestroy(rt.stream);
+    rt.d_partials = nullptr;
+    rt.h_pinned = nullptr;
+    rt.stream = nullptr;
+    rt.initialized = false;
+}
+
+hipStream_t stream() {
+    Runtime &rt = runtime();
+    if (!rt.initialized) runtime_init(0);
+    return rt.stream;
+}
+
+void sync() { QG_HIP(hipStreamSynchronize(stream())); }
+
+void *dev_alloc(size_t bytes) {
+    stream();
+    if (bytes == 0) bytes = 1;
+    Runtime &rt = runtime();
+    std::lock_guard<std::recursive_mutex> lk(rt.mtx);
+    if (bytes <= kCacheMaxBlock) {
+        size_t c = size_class(bytes);
+        auto it = free_lists().find(c);
+        if (it != free_lists().end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            g_cached -= c;
+            return p;
+        }
+        bytes = c;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // give cached blocks back and retry once
+        (void)hipGetLastError();
+        sync();
+        release_cache();
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        raise_error("Out of device memory (requested %zu bytes).", bytes);
+    }
+    return p;
+}
+
+void dev_free(void *ptr, size_t bytes) {
+    if (ptr == nullptr) return;
+    Runtime &rt = runtime();
+    std::lock_guard<std::recursive_mutex> lk(rt.mtx);
+    if (bytes == 0) bytes = 1;
+    if (bytes <= kCacheMaxBlock) {
+        size_t c = size_class(bytes);
+        if (g_cached + c <= kCacheMaxTotal) {
+            free_lists()[c].push_back(ptr);
+            g_cached += c;
+            return;
+        }
+    }
+    // the block may still be read by queued kernels: drain the stream first
+    QG_HIP(hipStreamSynchronize(rt.stream));
+    QG_HIP(hipFree(ptr));
+}
+
+size_t dev_cached_bytes() { return g_cached; }
+
+}  // namespace qg
