@@ -115,14 +115,16 @@ def roofline(runner, n, amp, reps=4):
     launch_bytes = 2 * amp * (1 << n)
     achieved = launch_bytes / (avg_ms * 1e-3) / 1e9
     split = {}
-    for name, rng in (('t<6', range(0, 6)), ('6<=t<13', range(6, 13)), ('t>=13', range(13, n))):
+    for name, rng in (('t=0 unit pair', range(0, 1)), ('1<=t<=6 wavefront exchange', range(1, 7)),
+                      ('t>=7 strided', range(7, n))):
         ms = float(np.mean([per_target[t] for t in rng]))
         split[name] = round(launch_bytes / (ms * 1e-3) / 1e9, 1)
     return {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-            'kernel': 'k_pairs (uncontrolled 2x2, complex64, 30 qubits)',
+            'kernel': 'uncontrolled 2x2 gate (k_unit_pair / k_wave_pair / k_strided_pair), complex64, 30 qubits',
             'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
-            'by_target_GBs': split}
+            'by_kernel_GBs': split,
+            'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
 
 
 def main():

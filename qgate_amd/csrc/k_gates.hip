@@ -5,11 +5,22 @@
 // (i0, i1 = i0 | 2^t) with all control bits of i0 set,
 //   a[i0] <- m00 a[i0] + m01 a[i1],  a[i1] <- m10 a[i0] + m11 a[i1].
 //
-// Layout: one interleaved complex array of 2^n amplitudes in HBM. Each work item moves
-// 16 bytes per side (2 complex64 or 1 complex128 amplitudes), so a wavefront touches 1 KiB
-// contiguous per side whenever the lowest inserted bit leaves room for it. The pair index
-// is scattered into the state index in closed form (insert zero bits), replacing the
-// reference's uploaded permutation tables.
+// Layout: one interleaved complex array of 2^n amplitudes in HBM, moved in 16-byte units
+// (W = 2 complex64 or 1 complex128 amplitudes) with non-temporal loads and stores. By where
+// the partner of a pair lies, a gate takes one of three streaming kernels:
+//   * inside the unit (complex64, target bit 0)            k_unit_pair: one load per pair;
+//   * inside the wavefront (partner 1..32 units away)      k_wave_pair: every lane loads its
+//     own unit (a fully coalesced 1 KiB per wavefront) and fetches the partner with a
+//     cross-lane exchange;
+//   * further away                                         k_strided_pair: one unit from each
+//     half, a wavefront covering 1 KiB contiguous on each side.
+// Diagonal gates with m00 == 1 (U1, S, T, Z, CZ, cu1) take k_phase and touch only the
+// amplitudes whose target and control bits are all 1.
+// Control bits above the unit are folded into the work-item index (zero bits inserted in
+// closed form, replacing the reference's uploaded permutation tables); control bits inside
+// a wavefront chunk or a unit are checked per lane or per amplitude.
+// Launch shape: one wavefront per block except for partners 64-128 units away (two); the
+// choice comes from per-target-bit bandwidth sweeps at 30 qubits.
 #include <algorithm>
 #include <type_traits>
 
@@ -20,12 +31,10 @@ namespace qg {
 
 namespace {
 
-constexpr int kBlock = 256;
-
-// NI > 0: compile-time count of inserted bits; NI == 0: count taken from b.n.
+// NI >= 0: compile-time count of inserted bits; NI == -1: count taken from b.n.
 template <int NI>
 __device__ __forceinline__ u64 scatter(u64 k, const BitInsert &b) {
-    if constexpr (NI == 0) {
+    if constexpr (NI < 0) {
         return insert_zero_bits(k, b);
     } else {
 #pragma unroll
@@ -38,12 +47,14 @@ __device__ __forceinline__ u64 scatter(u64 k, const BitInsert &b) {
 }
 
 template <class R>
-struct PairArgs {
+struct GateArgs {
     Mat2<R> m;
-    BitInsert ins;   // sorted control and target positions
-    u64 ctrl;        // control mask, OR-ed into every pair base
-    u64 tbit;        // target bit
-    u64 n_items;     // work items (W amplitudes per side each)
+    BitInsert ins;      // zero bits inserted into the work-item index
+    u64 item_set;       // bits OR-ed into the scattered index (controls; target for k_phase)
+    u64 n_items;
+    u64 dist;           // partner distance in units (strided) or lanes (wave)
+    u64 lane_ctrl;      // control bits of the unit index below the wavefront chunk (k_wave_pair)
+    unsigned amp_ctrl;  // control bits inside a unit (complex64 bit 0)
 };
 
 template <class R>
@@ -54,112 +65,104 @@ __device__ __forceinline__ void mix(const Mat2<R> &m, Cx<R> &x0, Cx<R> &x1) {
     x1 = y1;
 }
 
-// Pairs (i0, i0 | tbit); W consecutive amplitudes per side per item, U items per thread
-// with all loads issued before the first store.
-template <class R, int W, int NI, int U>
-__global__ __launch_bounds__(kBlock) void k_pairs(Cx<R> *__restrict__ a, const PairArgs<R> g) {
-    typedef Vec<R, W> V;
-    const u64 first = (u64)blockIdx.x * (kBlock * U) + threadIdx.x;
-    V x0[U], x1[U];
-    u64 i0[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-            i0[u] = scatter<NI>(w * W, g.ins) | g.ctrl;
-            x0[u] = *reinterpret_cast<const V *>(a + i0[u]);
-            x1[u] = *reinterpret_cast<const V *>(a + (i0[u] | g.tbit));
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-#pragma unroll
-            for (int j = 0; j < W; ++j) mix(g.m, x0[u].v[j], x1[u].v[j]);
-            *reinterpret_cast<V *>(a + i0[u]) = x0[u];
-            *reinterpret_cast<V *>(a + (i0[u] | g.tbit)) = x1[u];
-        }
-    }
+template <class R>
+__device__ __forceinline__ bool amp_on(int j, unsigned amp_ctrl) {
+    return ((unsigned)j & amp_ctrl) == amp_ctrl;
 }
 
-// Target bit 0 of a complex64 vector: the pair (i0, i0 + 1) is one 16-byte access.
-template <int NI, int U>
-__global__ __launch_bounds__(kBlock) void k_pairs_bit0(Cx<float> *__restrict__ a,
-                                                       const PairArgs<float> g) {
-    typedef Vec<float, 2> V;
-    const u64 first = (u64)blockIdx.x * (kBlock * U) + threadIdx.x;
-    V x[U];
-    u64 i0[U];
+// Partner in the other half, `dist` units away.
+template <class R, int NI, int B>
+__global__ __launch_bounds__(B) void k_strided_pair(Cx<R> *__restrict__ a, const GateArgs<R> g) {
+    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+    if (w >= g.n_items) return;
+    const u64 u0 = scatter<NI>(w, g.ins) | g.item_set, u1 = u0 | g.dist;
+    Amps<R> x0 = load_unit(a, u0);
+    Amps<R> x1 = load_unit(a, u1);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-            i0[u] = scatter<NI>(w, g.ins) | g.ctrl;
-            x[u] = *reinterpret_cast<const V *>(a + i0[u]);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-            mix(g.m, x[u].v[0], x[u].v[1]);
-            *reinterpret_cast<V *>(a + i0[u]) = x[u];
-        }
-    }
+    for (int j = 0; j < Unit<R>::W; ++j)
+        if (amp_on<R>(j, g.amp_ctrl)) mix(g.m, x0.v[j], x1.v[j]);
+    store_unit(a, u0, x0);
+    store_unit(a, u1, x1);
 }
 
 template <class R>
-struct PhaseArgs {
-    Cx<R> d;         // phase applied where every bit of `set` is 1
-    BitInsert ins;
-    u64 set;
-    u64 n_items;
-};
-
-// Diagonal gate with m00 == 1 (U1, S, T, Z, CZ, cu1, ...): only amplitudes whose target and
-// control bits are all 1 change, so half (or less) of the vector is read and written.
-template <class R, int W, int NI, int U>
-__global__ __launch_bounds__(kBlock) void k_phase(Cx<R> *__restrict__ a, const PhaseArgs<R> g) {
-    typedef Vec<R, W> V;
-    const u64 first = (u64)blockIdx.x * (kBlock * U) + threadIdx.x;
-    V x[U];
-    u64 idx[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-            idx[u] = scatter<NI>(w * W, g.ins) | g.set;
-            x[u] = *reinterpret_cast<const V *>(a + idx[u]);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const u64 w = first + (u64)u * kBlock;
-        if (w < g.n_items) {
-#pragma unroll
-            for (int j = 0; j < W; ++j) x[u].v[j] = cmul(g.d, x[u].v[j]);
-            *reinterpret_cast<V *>(a + idx[u]) = x[u];
-        }
-    }
+__device__ __forceinline__ Cx<R> shfl_xor_cx(const Cx<R> &v, int d) {
+    return cx<R>(__shfl_xor(v.re, d, 64), __shfl_xor(v.im, d, 64));
 }
 
-template <int U>
-inline unsigned grid_for(u64 n_items) {
-    const u64 per_block = (u64)kBlock * U;
-    return (unsigned)((n_items + per_block - 1) / per_block);
+// Partner in the same wavefront, `dist` (1..32) lanes away. One 64-unit chunk per wavefront.
+template <class R, int NI>
+__global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const GateArgs<R> g) {
+    const u64 w = blockIdx.x;   // grid == n_items chunks
+    const int lane = threadIdx.x;
+    const u64 unit = ((scatter<NI>(w, g.ins) | g.item_set) << 6) | (u64)lane;
+    const Amps<R> x = load_unit(a, unit);
+    Amps<R> p;
+#pragma unroll
+    for (int j = 0; j < Unit<R>::W; ++j) p.v[j] = shfl_xor_cx(x.v[j], (int)g.dist);
+    if (((u64)lane & g.lane_ctrl) != g.lane_ctrl) return;
+    const bool hi = ((u64)lane & g.dist) != 0;
+    Amps<R> y = x;
+#pragma unroll
+    for (int j = 0; j < Unit<R>::W; ++j) {
+        if (!amp_on<R>(j, g.amp_ctrl)) continue;
+        Cx<R> lo_v = hi ? p.v[j] : x.v[j];
+        Cx<R> hi_v = hi ? x.v[j] : p.v[j];
+        mix(g.m, lo_v, hi_v);
+        y.v[j] = hi ? hi_v : lo_v;
+    }
+    store_unit(a, unit, y);
 }
 
-// Calls f(std::integral_constant<int, NI>) with NI = n for n <= 4, else 0 (dynamic).
+// Target bit 0 of a complex64 vector: the pair is one unit.
+template <int NI, int B>
+__global__ __launch_bounds__(B) void k_unit_pair(Cx<float> *__restrict__ a, const GateArgs<float> g) {
+    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+    if (w >= g.n_items) return;
+    const u64 u = scatter<NI>(w, g.ins) | g.item_set;
+    Amps<float> x = load_unit(a, u);
+    mix(g.m, x.v[0], x.v[1]);
+    store_unit(a, u, x);
+}
+
+// Diagonal with m00 == 1: multiply the amplitudes whose target and control bits are 1.
+template <class R, int NI, int B>
+__global__ __launch_bounds__(B) void k_phase(Cx<R> *__restrict__ a, const GateArgs<R> g) {
+    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+    if (w >= g.n_items) return;
+    const u64 u = scatter<NI>(w, g.ins) | g.item_set;
+    Amps<R> x = load_unit(a, u);
+#pragma unroll
+    for (int j = 0; j < Unit<R>::W; ++j)
+        if (amp_on<R>(j, g.amp_ctrl)) x.v[j] = cmul(g.m.m11, x.v[j]);
+    store_unit(a, u, x);
+}
+
+inline unsigned grid_of(u64 n_items, int block) {
+    return (unsigned)((n_items + block - 1) / block);
+}
+
+// Calls f(std::integral_constant<int, NI>) with NI = n for n <= 4, else -1 (dynamic).
 template <class F>
 void with_ni(int n, F &&f) {
     switch (n) {
+    case 0: f(std::integral_constant<int, 0>()); break;
     case 1: f(std::integral_constant<int, 1>()); break;
     case 2: f(std::integral_constant<int, 2>()); break;
     case 3: f(std::integral_constant<int, 3>()); break;
     case 4: f(std::integral_constant<int, 4>()); break;
-    default: f(std::integral_constant<int, 0>()); break;
+    default: f(std::integral_constant<int, -1>()); break;
     }
+}
+
+int log2_of(int w) { return w == 2 ? 1 : 0; }
+
+BitInsert make_insert(std::vector<int> pos) {
+    std::sort(pos.begin(), pos.end());
+    BitInsert b;
+    b.n = (int)pos.size();
+    for (int i = 0; i < b.n; ++i) b.pos[i] = pos[i];
+    return b;
 }
 
 }  // namespace
@@ -167,68 +170,102 @@ void with_ni(int n, F &&f) {
 template <class R>
 void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<int> &ctrl,
                  int target) {
-    constexpr int WV = Unit<R>::W;  // amplitudes per 16-byte access
-    constexpr int U = 4;
+    constexpr int W = Unit<R>::W;
+    const int lw = log2_of(W);                 // amplitude bits inside a unit
     QG_REQUIRE(0 <= target && target < n_lanes, "target lane %d out of range", target);
-    std::vector<int> pos(ctrl);
-    pos.push_back(target);
-    std::sort(pos.begin(), pos.end());
-    for (size_t i = 0; i < pos.size(); ++i) {
-        QG_REQUIRE(0 <= pos[i] && pos[i] < n_lanes, "lane %d out of range", pos[i]);
-        QG_REQUIRE(i == 0 || pos[i] != pos[i - 1], "control and target lanes overlap");
+    std::vector<int> all(ctrl);
+    all.push_back(target);
+    std::sort(all.begin(), all.end());
+    for (size_t i = 0; i < all.size(); ++i) {
+        QG_REQUIRE(0 <= all[i] && all[i] < n_lanes, "lane %d out of range", all[i]);
+        QG_REQUIRE(i == 0 || all[i] != all[i - 1], "control and target lanes overlap");
     }
-    BitInsert ins;
-    ins.n = (int)pos.size();
-    for (int i = 0; i < ins.n; ++i) ins.pos[i] = pos[i];
-    u64 ctrl_mask = 0;
-    for (int c : ctrl) ctrl_mask |= u64(1) << c;
-    const u64 tbit = u64(1) << target;
-    const int n_free = n_lanes - ins.n;
+    const int unit_bits = n_lanes - lw;        // bits of the unit index
     hipStream_t s = stream();
 
+    GateArgs<R> g;
+    g.m = round_matrix<R>(gm);
+    g.amp_ctrl = 0;
+    g.lane_ctrl = 0;
+    g.dist = 0;
+    g.item_set = 0;
+    // controls inside a unit (complex64 bit 0)
+    std::vector<int> unit_ctrl;                // control positions in the unit index
+    for (int c : ctrl) {
+        if (c < lw)
+            g.amp_ctrl |= 1u << c;
+        else
+            unit_ctrl.push_back(c - lw);
+    }
+
     const auto &m = gm.m;
-    const bool offdiag_zero = m[0][1] == 0. && m[1][0] == 0.;
-    if (offdiag_zero && m[0][0] == 1.) {
-        if (m[1][1] == 1.) return;  // identity on the touched subspace
-        PhaseArgs<R> g;
-        g.d = cx<R>((R)m[1][1].real(), (R)m[1][1].imag());
-        g.ins = ins;
-        g.set = ctrl_mask | tbit;
-        const bool vec = WV > 1 && pos[0] >= 1;
-        g.n_items = (u64(1) << n_free) / (vec ? WV : 1);
-        with_ni(ins.n, [&](auto ni) {
+    const bool phase = m[0][1] == 0. && m[1][0] == 0. && m[0][0] == 1.;
+    if (phase) {
+        if (m[1][1] == 1.) return;                             // identity on the touched subspace
+        std::vector<int> pos(unit_ctrl);
+        if (target < lw)
+            g.amp_ctrl |= 1u << target;
+        else
+            pos.push_back(target - lw);
+        g.ins = make_insert(pos);
+        for (int p : pos) g.item_set |= u64(1) << p;
+        g.n_items = u64(1) << (unit_bits - (int)pos.size());
+        with_ni((int)pos.size(), [&](auto ni) {
             constexpr int NI = decltype(ni)::value;
-            if (vec)
-                hipLaunchKernelGGL((k_phase<R, WV, NI, U>), grid_for<U>(g.n_items), kBlock, 0, s, a, g);
-            else
-                hipLaunchKernelGGL((k_phase<R, 1, NI, U>), grid_for<U>(g.n_items), kBlock, 0, s, a, g);
+            hipLaunchKernelGGL((k_phase<R, NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
         });
         QG_HIP(hipGetLastError());
         return;
     }
 
-    PairArgs<R> g;
-    g.m = round_matrix<R>(gm);
-    g.ins = ins;
-    g.ctrl = ctrl_mask;
-    g.tbit = tbit;
-    const u64 n_pairs = u64(1) << n_free;
-    with_ni(ins.n, [&](auto ni) {
+    if constexpr (std::is_same<R, float>::value) {
+        if (target == 0) {                                     // pair inside the unit
+            g.ins = make_insert(unit_ctrl);
+            for (int p : unit_ctrl) g.item_set |= u64(1) << p;
+            g.n_items = u64(1) << (unit_bits - (int)unit_ctrl.size());
+            with_ni((int)unit_ctrl.size(), [&](auto ni) {
+                constexpr int NI = decltype(ni)::value;
+                hipLaunchKernelGGL((k_unit_pair<NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
+            });
+            QG_HIP(hipGetLastError());
+            return;
+        }
+    }
+
+    const int tu = target - lw;                                // target bit of the unit index
+    if (tu < 6 && unit_bits >= 6) {                            // partner in the wavefront
+        std::vector<int> chunk_ctrl;
+        for (int p : unit_ctrl) {
+            if (p >= 6)
+                chunk_ctrl.push_back(p - 6);
+            else
+                g.lane_ctrl |= u64(1) << p;
+        }
+        g.ins = make_insert(chunk_ctrl);
+        for (int p : chunk_ctrl) g.item_set |= u64(1) << p;
+        g.dist = u64(1) << tu;
+        g.n_items = u64(1) << (unit_bits - 6 - (int)chunk_ctrl.size());
+        with_ni((int)chunk_ctrl.size(), [&](auto ni) {
+            constexpr int NI = decltype(ni)::value;
+            hipLaunchKernelGGL((k_wave_pair<R, NI>), (unsigned)g.n_items, 64, 0, s, a, g);
+        });
+        QG_HIP(hipGetLastError());
+        return;
+    }
+
+    std::vector<int> pos(unit_ctrl);
+    pos.push_back(tu);
+    g.ins = make_insert(pos);
+    for (int p : unit_ctrl) g.item_set |= u64(1) << p;
+    g.dist = u64(1) << tu;
+    g.n_items = u64(1) << (unit_bits - (int)pos.size());
+    const bool two_waves = tu == 6 || tu == 7;
+    with_ni((int)pos.size(), [&](auto ni) {
         constexpr int NI = decltype(ni)::value;
-        if constexpr (std::is_same<R, float>::value) {
-            if (target == 0) {
-                g.n_items = n_pairs;
-                hipLaunchKernelGGL((k_pairs_bit0<NI, U>), grid_for<U>(g.n_items), kBlock, 0, s, a, g);
-                return;
-            }
-        }
-        if (WV > 1 && pos[0] >= 1) {
-            g.n_items = n_pairs / WV;
-            hipLaunchKernelGGL((k_pairs<R, WV, NI, U>), grid_for<U>(g.n_items), kBlock, 0, s, a, g);
-        } else {
-            g.n_items = n_pairs;
-            hipLaunchKernelGGL((k_pairs<R, 1, NI, U>), grid_for<U>(g.n_items), kBlock, 0, s, a, g);
-        }
+        if (two_waves)
+            hipLaunchKernelGGL((k_strided_pair<R, NI, 128>), grid_of(g.n_items, 128), 128, 0, s, a, g);
+        else
+            hipLaunchKernelGGL((k_strided_pair<R, NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
     });
     QG_HIP(hipGetLastError());
 }
@@ -241,6 +278,7 @@ template void launch_gate<double>(Cx<double> *, int, const GateMatrix &, const s
 namespace {
 
 constexpr int kMaxFuse = 6;
+constexpr int kBlock = 256;
 
 template <class R>
 struct UnitaryArgs {

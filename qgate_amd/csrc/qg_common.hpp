@@ -31,6 +31,44 @@ struct Unit {
     static constexpr int W = 16 / (2 * (int)sizeof(R));
 };
 
+// 16-byte access unit as a vector type (the form the non-temporal builtins take).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <class R>
+struct UnitVec;
+template <>
+struct UnitVec<float> {
+    typedef f32x4 T;
+};
+template <>
+struct UnitVec<double> {
+    typedef f64x2 T;
+};
+
+// One 16-byte unit of amplitudes (W = Unit<R>::W of them), loaded / stored with streaming
+// (non-temporal) hints: state-vector passes touch every byte once, so they bypass caching.
+template <class R>
+struct Amps {
+    Cx<R> v[Unit<R>::W];
+};
+
+template <class R>
+__device__ __forceinline__ Amps<R> load_unit(const Cx<R> *a, u64 unit) {
+    typedef typename UnitVec<R>::T T;
+    T t = __builtin_nontemporal_load(reinterpret_cast<const T *>(a) + unit);
+    Amps<R> r;
+    __builtin_memcpy(&r, &t, sizeof(T));
+    return r;
+}
+
+template <class R>
+__device__ __forceinline__ void store_unit(Cx<R> *a, u64 unit, const Amps<R> &r) {
+    typedef typename UnitVec<R>::T T;
+    T t;
+    __builtin_memcpy(&t, &r, sizeof(T));
+    __builtin_nontemporal_store(t, reinterpret_cast<T *>(a) + unit);
+}
+
 template <class R>
 struct Mat2 {
     Cx<R> m00, m01, m10, m11;
