@@ -15,6 +15,10 @@ roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, t
 HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
 launch time, against 8 TB/s.
 
+adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qubit, 12-layer
+RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
+once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
+
 cpu_baseline: the reference's own C++ CPU runtime (oracle/_ref, or the C restatement when
 that is not built) applying the first gates of the same circuit to a 30-qubit complex64
 state in host memory, for about 15 s of work, all host cores.
@@ -127,6 +131,34 @@ def roofline(runner, n, amp, reps=4):
             'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
 
 
+def adjoint_timing(dtype, reps=3):
+    """Forward (<H>) and backward (full gradient) times of the 24-qubit, 12-layer ansatz
+    (576 parameters; H = sum a_i Z_i Z_i+1 + sum b_i X_i, a, b ~ N(0, 1), seed 4)."""
+    from qgate_amd import workloads
+    from qgate_amd.adjoint import PauliSum, hea_circuit
+    n, layers = 24, 12
+    params = workloads.hea_params(n, layers, seed=4)
+    a, b = workloads.hea_observable(n, seed=4)
+    h = PauliSum.zz_x(a, b)
+    c = hea_circuit(n, layers, dtype)
+    c.expectation(params, h)
+    c.backward(params)
+    fwd, bwd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        c.expectation(params, h)
+        t1 = time.perf_counter()
+        c.backward(params)
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    c.delete()
+    f, g = float(np.median(fwd)), float(np.median(bwd))
+    return {'forward_ms': round(f * 1e3, 2), 'backward_ms': round(g * 1e3, 2),
+            'backward_over_forward': round(g / f, 3), 'gradient_over_forward': round((f + g) / f, 3),
+            'gates': 3 * n * layers, 'params': 2 * n * layers}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -190,6 +222,11 @@ def main():
     r.delete()
     print('[bench] roofline: {:.1f} GB/s'.format(roof['achieved']), file=sys.stderr, flush=True)
 
+    adjoint = {}
+    for name, dt in (('complex64', np.float32), ('complex128', np.float64)):
+        adjoint[name] = adjoint_timing(dt)
+        print('[bench] adjoint 24q {}: {}'.format(name, adjoint[name]), file=sys.stderr, flush=True)
+
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'GB/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(step_s * 1e3, 2),
@@ -204,6 +241,7 @@ def main():
         'pool_build_ms': round((t3 - t2) * 1e3, 2),
         'pool_2p20_draws_ms': round((t4 - t3) * 1e3, 2),
         'measure_all_ms': round((t5 - t4) * 1e3, 2),
+        'adjoint_24q': adjoint,
         'roofline': roof,
     }
     if not args.no_cpu:

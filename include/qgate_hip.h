@@ -172,6 +172,46 @@ int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations,
 /* sampling_pool_delete (glue.cpp:592-602). */
 int qghip_sampling_pool_delete(qghip_sampling_pool *pool);
 
+/* ---- adjoint differentiation (no reference counterpart) ------------------------------ */
+
+/* A parameterised circuit on n_qubits qubits (qubit q = bit q of the amplitude index) for
+ * expectation values <psi|H|psi> of Pauli sums and their gradients by adjoint (reverse-mode)
+ * differentiation: two device-resident state vectors, no checkpoints. Callers: autograd
+ * wrappers and variational workloads. */
+typedef struct qghip_circuit qghip_circuit;
+int qghip_circuit_new(int prec, int n_qubits, qghip_circuit **out);
+int qghip_circuit_delete(qghip_circuit *c);
+/* Append a fixed (controlled) 2x2 gate, FP64 row-major (re, im) pairs. */
+int qghip_circuit_add_gate(qghip_circuit *c, const double *mat8, const int *ctrl, int n_ctrl,
+                           int target);
+/* Append a (controlled) rotation exp(-i theta G / 2), theta = params[param_index],
+ * G = X (pauli 1), Y (2) or Z (3): qgate's RX / RY / RZ. */
+int qghip_circuit_add_rotation(qghip_circuit *c, int pauli, int param_index, const int *ctrl,
+                               int n_ctrl, int target);
+int qghip_circuit_n_params(const qghip_circuit *c, int *n_params);
+/* <H> for H = sum_k coefs[k] P_k, P_k the Pauli string with X on xmask[k] bits, Z on
+ * zmask[k] bits (both bits: Y). Leaves the final state and H|psi> on the device. */
+int qghip_circuit_expectation(qghip_circuit *c, const double *params, int n_params, int n_terms,
+                              const double *coefs, const uint64_t *xmask, const uint64_t *zmask,
+                              double *value);
+/* d<H>/dparams for the observable of the last expectation call (re-running the forward
+ * pass if `params` differ from that call's); grad has n_params entries. */
+int qghip_circuit_backward(qghip_circuit *c, const double *params, int n_params, double *grad);
+/* expectation + backward. */
+int qghip_circuit_gradient(qghip_circuit *c, const double *params, int n_params, int n_terms,
+                           const double *coefs, const uint64_t *xmask, const uint64_t *zmask,
+                           double *value, double *grad);
+/* Copy of the final state of the last expectation call (2^n complex amplitudes). */
+int qghip_circuit_state(qghip_circuit *c, void *out, int64_t array_size);
+
+/* ---- launch shape (no reference counterpart) ----------------------------------------- */
+
+/* Kernels over more than `items` work items (one per amplitude, unit or pair) are launched
+ * in pieces of `items` (a power of two in [2^12, 2^30]; default 2^30), keeping every grid
+ * below the 2^32 work-item limit at 33+ qubits. Results do not depend on it; tests lower it
+ * to run the piecewise path on small states. */
+int qghip_set_launch_piece(int64_t items);
+
 /* ---- measurement helper (no reference counterpart) ------------------------------------ */
 
 /* Time `reps` applications of one (controlled) gate with HIP events on the library stream;

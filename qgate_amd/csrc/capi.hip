@@ -36,8 +36,6 @@ struct qghip_sampling_pool {
 
 namespace {
 
-thread_local std::string g_last_error;
-
 struct ValueError : Error {
     using Error::Error;
 };
@@ -51,13 +49,13 @@ int guarded(F &&f) {
         f();
         return QGHIP_OK;
     } catch (const ValueError &e) {
-        g_last_error = e.what();
+        last_error() = e.what();
         return QGHIP_EVALUE;
     } catch (const std::exception &e) {
-        g_last_error = e.what();
+        last_error() = e.what();
         return QGHIP_ERROR;
     } catch (...) {
-        g_last_error = "unknown error";
+        last_error() = "unknown error";
         return QGHIP_ERROR;
     }
 }
@@ -145,7 +143,7 @@ std::vector<StateView> make_views(const int *lane_tables, qghip_qubit_states *co
 
 extern "C" {
 
-const char *qghip_last_error(void) { return g_last_error.c_str(); }
+const char *qghip_last_error(void) { return last_error().c_str(); }
 
 int qghip_devices_initialize(int device_id) {
     return guarded([&] { runtime_init(device_id); });
@@ -153,6 +151,14 @@ int qghip_devices_initialize(int device_id) {
 
 int qghip_devices_clear(void) {
     return guarded([&] { runtime_clear(); });
+}
+
+int qghip_set_launch_piece(int64_t items) {
+    return guarded([&] {
+        if (items < (int64_t(1) << 12) || items > (int64_t(1) << 30) || (items & (items - 1)) != 0)
+            value_error("launch piece must be a power of two in [2^12, 2^30]");
+        set_launch_piece_items((u64)items);
+    });
 }
 
 int qghip_device_info(int *device_id, int *n_cus, uint64_t *total_mem) {

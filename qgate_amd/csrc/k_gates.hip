@@ -72,8 +72,8 @@ __device__ __forceinline__ bool amp_on(int j, unsigned amp_ctrl) {
 
 // Partner in the other half, `dist` units away.
 template <class R, int NI, int B>
-__global__ __launch_bounds__(B) void k_strided_pair(Cx<R> *__restrict__ a, const GateArgs<R> g) {
-    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+__global__ __launch_bounds__(B) void k_strided_pair(Cx<R> *__restrict__ a, const GateArgs<R> g, u64 first) {
+    const u64 w = first + (u64)blockIdx.x * B + threadIdx.x;
     if (w >= g.n_items) return;
     const u64 u0 = scatter<NI>(w, g.ins) | g.item_set, u1 = u0 | g.dist;
     Amps<R> x0 = load_unit(a, u0);
@@ -92,8 +92,8 @@ __device__ __forceinline__ Cx<R> shfl_xor_cx(const Cx<R> &v, int d) {
 
 // Partner in the same wavefront, `dist` (1..32) lanes away. One 64-unit chunk per wavefront.
 template <class R, int NI>
-__global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const GateArgs<R> g) {
-    const u64 w = blockIdx.x;   // grid == n_items chunks
+__global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const GateArgs<R> g, u64 first) {
+    const u64 w = first + blockIdx.x;   // one block per 64-unit chunk
     const int lane = threadIdx.x;
     const u64 unit = ((scatter<NI>(w, g.ins) | g.item_set) << 6) | (u64)lane;
     const Amps<R> x = load_unit(a, unit);
@@ -116,8 +116,8 @@ __global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const G
 
 // Target bit 0 of a complex64 vector: the pair is one unit.
 template <int NI, int B>
-__global__ __launch_bounds__(B) void k_unit_pair(Cx<float> *__restrict__ a, const GateArgs<float> g) {
-    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+__global__ __launch_bounds__(B) void k_unit_pair(Cx<float> *__restrict__ a, const GateArgs<float> g, u64 first) {
+    const u64 w = first + (u64)blockIdx.x * B + threadIdx.x;
     if (w >= g.n_items) return;
     const u64 u = scatter<NI>(w, g.ins) | g.item_set;
     Amps<float> x = load_unit(a, u);
@@ -127,8 +127,8 @@ __global__ __launch_bounds__(B) void k_unit_pair(Cx<float> *__restrict__ a, cons
 
 // Diagonal with m00 == 1: multiply the amplitudes whose target and control bits are 1.
 template <class R, int NI, int B>
-__global__ __launch_bounds__(B) void k_phase(Cx<R> *__restrict__ a, const GateArgs<R> g) {
-    const u64 w = (u64)blockIdx.x * B + threadIdx.x;
+__global__ __launch_bounds__(B) void k_phase(Cx<R> *__restrict__ a, const GateArgs<R> g, u64 first) {
+    const u64 w = first + (u64)blockIdx.x * B + threadIdx.x;
     if (w >= g.n_items) return;
     const u64 u = scatter<NI>(w, g.ins) | g.item_set;
     Amps<R> x = load_unit(a, u);
@@ -136,10 +136,6 @@ __global__ __launch_bounds__(B) void k_phase(Cx<R> *__restrict__ a, const GateAr
     for (int j = 0; j < Unit<R>::W; ++j)
         if (amp_on<R>(j, g.amp_ctrl)) x.v[j] = cmul(g.m.m11, x.v[j]);
     store_unit(a, u, x);
-}
-
-inline unsigned grid_of(u64 n_items, int block) {
-    return (unsigned)((n_items + block - 1) / block);
 }
 
 // Calls f(std::integral_constant<int, NI>) with NI = n for n <= 4, else -1 (dynamic).
@@ -212,7 +208,9 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
         g.n_items = u64(1) << (unit_bits - (int)pos.size());
         with_ni((int)pos.size(), [&](auto ni) {
             constexpr int NI = decltype(ni)::value;
-            hipLaunchKernelGGL((k_phase<R, NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
+            for_each_piece(g.n_items, [&](u64 first, u64 count) {
+                hipLaunchKernelGGL((k_phase<R, NI, 64>), grid_for(count, 64), 64, 0, s, a, g, first);
+            });
         });
         QG_HIP(hipGetLastError());
         return;
@@ -225,7 +223,9 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
             g.n_items = u64(1) << (unit_bits - (int)unit_ctrl.size());
             with_ni((int)unit_ctrl.size(), [&](auto ni) {
                 constexpr int NI = decltype(ni)::value;
-                hipLaunchKernelGGL((k_unit_pair<NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
+                for_each_piece(g.n_items, [&](u64 first, u64 count) {
+                    hipLaunchKernelGGL((k_unit_pair<NI, 64>), grid_for(count, 64), 64, 0, s, a, g, first);
+                });
             });
             QG_HIP(hipGetLastError());
             return;
@@ -247,7 +247,10 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
         g.n_items = u64(1) << (unit_bits - 6 - (int)chunk_ctrl.size());
         with_ni((int)chunk_ctrl.size(), [&](auto ni) {
             constexpr int NI = decltype(ni)::value;
-            hipLaunchKernelGGL((k_wave_pair<R, NI>), (unsigned)g.n_items, 64, 0, s, a, g);
+            // pieces in chunks: 64 work items each
+            for_each_piece(g.n_items << 6, [&](u64 first, u64 count) {
+                hipLaunchKernelGGL((k_wave_pair<R, NI>), (unsigned)(count >> 6), 64, 0, s, a, g, first >> 6);
+            });
         });
         QG_HIP(hipGetLastError());
         return;
@@ -262,10 +265,12 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
     const bool two_waves = tu == 6 || tu == 7;
     with_ni((int)pos.size(), [&](auto ni) {
         constexpr int NI = decltype(ni)::value;
-        if (two_waves)
-            hipLaunchKernelGGL((k_strided_pair<R, NI, 128>), grid_of(g.n_items, 128), 128, 0, s, a, g);
-        else
-            hipLaunchKernelGGL((k_strided_pair<R, NI, 64>), grid_of(g.n_items, 64), 64, 0, s, a, g);
+        for_each_piece(g.n_items, [&](u64 first, u64 count) {
+            if (two_waves)
+                hipLaunchKernelGGL((k_strided_pair<R, NI, 128>), grid_for(count, 128), 128, 0, s, a, g, first);
+            else
+                hipLaunchKernelGGL((k_strided_pair<R, NI, 64>), grid_for(count, 64), 64, 0, s, a, g, first);
+        });
     });
     QG_HIP(hipGetLastError());
 }
@@ -292,12 +297,12 @@ struct UnitaryArgs {
 template <class R, int K>
 __global__ __launch_bounds__(kBlock) void k_unitary(Cx<R> *__restrict__ a,
                                                     const Cx<R> *__restrict__ umat,
-                                                    const UnitaryArgs<R> g) {
+                                                    const UnitaryArgs<R> g, u64 first) {
     constexpr int D = 1 << K;
     __shared__ Cx<R> su[D * D];
     for (int i = threadIdx.x; i < D * D; i += kBlock) su[i] = umat[i];
     __syncthreads();
-    const u64 grp = (u64)blockIdx.x * kBlock + threadIdx.x;
+    const u64 grp = first + (u64)blockIdx.x * kBlock + threadIdx.x;
     if (grp >= g.n_groups) return;
     const u64 base = insert_zero_bits(grp, g.ins);
     Cx<R> x[D];
@@ -344,15 +349,17 @@ void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
     Cx<R> *dm = static_cast<Cx<R> *>(dev_alloc(sizeof(Cx<R>) * D * D));
     hipStream_t s = stream();
     QG_HIP(hipMemcpyAsync(dm, hm.data(), sizeof(Cx<R>) * D * D, hipMemcpyHostToDevice, s));
-    const unsigned grid = (unsigned)((g.n_groups + kBlock - 1) / kBlock);
-    switch (k) {
-    case 1: hipLaunchKernelGGL((k_unitary<R, 1>), grid, kBlock, 0, s, a, dm, g); break;
-    case 2: hipLaunchKernelGGL((k_unitary<R, 2>), grid, kBlock, 0, s, a, dm, g); break;
-    case 3: hipLaunchKernelGGL((k_unitary<R, 3>), grid, kBlock, 0, s, a, dm, g); break;
-    case 4: hipLaunchKernelGGL((k_unitary<R, 4>), grid, kBlock, 0, s, a, dm, g); break;
-    case 5: hipLaunchKernelGGL((k_unitary<R, 5>), grid, kBlock, 0, s, a, dm, g); break;
-    case 6: hipLaunchKernelGGL((k_unitary<R, 6>), grid, kBlock, 0, s, a, dm, g); break;
-    }
+    for_each_piece(g.n_groups, [&](u64 first, u64 count) {
+        const unsigned grid = grid_for(count, kBlock);
+        switch (k) {
+        case 1: hipLaunchKernelGGL((k_unitary<R, 1>), grid, kBlock, 0, s, a, dm, g, first); break;
+        case 2: hipLaunchKernelGGL((k_unitary<R, 2>), grid, kBlock, 0, s, a, dm, g, first); break;
+        case 3: hipLaunchKernelGGL((k_unitary<R, 3>), grid, kBlock, 0, s, a, dm, g, first); break;
+        case 4: hipLaunchKernelGGL((k_unitary<R, 4>), grid, kBlock, 0, s, a, dm, g, first); break;
+        case 5: hipLaunchKernelGGL((k_unitary<R, 5>), grid, kBlock, 0, s, a, dm, g, first); break;
+        case 6: hipLaunchKernelGGL((k_unitary<R, 6>), grid, kBlock, 0, s, a, dm, g, first); break;
+        }
+    });
     QG_HIP(hipGetLastError());
     QG_HIP(hipStreamSynchronize(s));   // hm is a host temporary
     dev_free(dm, sizeof(Cx<R>) * D * D);

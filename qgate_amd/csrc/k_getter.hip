@@ -74,8 +74,8 @@ __global__ __launch_bounds__(kBlock) void k_get_states(void *out, const ViewArgs
 // out[x] = sum_h prod_q |a_q(x | h << n_lanes)|^2, FP64 accumulation.
 template <class R, class O>
 __global__ __launch_bounds__(kBlock) void k_prob_array(O *out, const ViewArgs g, int n_lanes,
-                                                       u64 n_hidden_states, u64 n_out) {
-    const u64 x = (u64)blockIdx.x * kBlock + threadIdx.x;
+                                                       u64 n_hidden_states, u64 n_out, u64 first) {
+    const u64 x = first + (u64)blockIdx.x * kBlock + threadIdx.x;
     if (x >= n_out) return;
     double s = 0.;
     for (u64 h = 0; h < n_hidden_states; ++h) {
@@ -163,12 +163,14 @@ void launch_prob_array(void *d_out, bool out_double, const std::vector<StateView
     const u64 n_out = u64(1) << n_lanes;
     const u64 n_h = u64(1) << n_hidden;
     hipStream_t s = stream();
-    if (out_double)
-        hipLaunchKernelGGL((k_prob_array<R, double>), blocks_for(n_out), kBlock, 0, s,
-                           static_cast<double *>(d_out), g, n_lanes, n_h, n_out);
-    else
-        hipLaunchKernelGGL((k_prob_array<R, R>), blocks_for(n_out), kBlock, 0, s,
-                           static_cast<R *>(d_out), g, n_lanes, n_h, n_out);
+    for_each_piece(n_out, [&](u64 first, u64 count) {
+        if (out_double)
+            hipLaunchKernelGGL((k_prob_array<R, double>), blocks_for(count), kBlock, 0, s,
+                               static_cast<double *>(d_out), g, n_lanes, n_h, n_out, first);
+        else
+            hipLaunchKernelGGL((k_prob_array<R, R>), blocks_for(count), kBlock, 0, s,
+                               static_cast<R *>(d_out), g, n_lanes, n_h, n_out, first);
+    });
     QG_HIP(hipGetLastError());
 }
 
@@ -433,8 +435,10 @@ void pool_sample(const PoolData &pd, i64 *host_obs, i64 n_samples, const double 
     i64 *d_obs = reinterpret_cast<i64 *>(buf + bytes);
     try {
         QG_HIP(hipMemcpyAsync(d_u, host_u, bytes, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_pool_sample, blocks_for((u64)n_samples), kBlock, 0, s, g, d_u, d_obs,
-                           n_samples);
+        for_each_piece((u64)n_samples, [&](u64 first, u64 count) {
+            hipLaunchKernelGGL(k_pool_sample, blocks_for(count), kBlock, 0, s, g, d_u + first, d_obs + first,
+                               (i64)count);
+        });
         QG_HIP(hipGetLastError());
         QG_HIP(hipMemcpyAsync(host_obs, d_obs, bytes, hipMemcpyDeviceToHost, s));
         QG_HIP(hipStreamSynchronize(s));

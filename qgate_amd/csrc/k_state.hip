@@ -7,7 +7,12 @@
 //   decohere              :217-246  kept half *= 1/sqrt(p) (or 1/sqrt(1-p)), other half = 0
 //   decohereAndSeparate   :248-284  kept half * norm -> new (n-1)-lane vector
 //   applyReset            :286-305  a[i0] = a[i1], a[i1] = 0
-// Probability sums are accumulated in FP64 whatever the state precision.
+// Every pass streams 16-byte units (2 complex64 or 1 complex128 amplitudes) with
+// non-temporal hints. Probability sums accumulate in FP64 whatever the state precision, in a
+// fixed order (per-block partials, then one block over the partials), so repeated calls on
+// the same state return the same bits.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
 
@@ -15,23 +20,19 @@ namespace qg {
 
 namespace {
 
+constexpr int kWave = 64;
 constexpr int kBlock = 256;
 
-inline unsigned blocks_for(u64 n, int per_thread = 1) {
-    const u64 per_block = (u64)kBlock * per_thread;
-    return (unsigned)((n + per_block - 1) / per_block);
-}
-
-// a[i] = (i == 0), written as 16-byte units.
-template <class R, int W>
-__global__ __launch_bounds__(kBlock) void k_zero_state(Cx<R> *a, u64 n_units) {
-    const u64 u = (u64)blockIdx.x * kBlock + threadIdx.x;
+// a[i] = (i == 0)
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_zero_state(Cx<R> *a, u64 n_units, u64 first) {
+    const u64 u = first + (u64)blockIdx.x * kBlock + threadIdx.x;
     if (u >= n_units) return;
-    Vec<R, W> v;
+    Amps<R> v;
 #pragma unroll
-    for (int j = 0; j < W; ++j) v.v[j] = cx<R>(0, 0);
+    for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(0, 0);
     if (u == 0) v.v[0].re = R(1);
-    reinterpret_cast<Vec<R, W> *>(a)[u] = v;
+    store_unit(a, u, v);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -40,87 +41,123 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block sum of `v` (fixed order); valid in thread 0.
+// Block sum of `v` in a fixed order; valid in thread 0.
+template <int B>
 __device__ __forceinline__ double block_sum(double v) {
-    __shared__ double part[kBlock / 64];
+    __shared__ double part[B / 64];
     v = wave_sum(v);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (lane == 0) part[wave] = v;
     __syncthreads();
     double s = 0.;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
+        for (int i = 0; i < B / 64; ++i) s += part[i];
     return s;
 }
 
-// Partial sums of |a_i|^2 over the half with bit `lane` = 0; grid-stride over W-wide units.
-template <class R, int W>
+constexpr int kProbUnitsPerThread = 16;
+
+// Partial sums of |a|^2 over the units of the half with bit `lane` = 0 (lane at unit level:
+// a zero bit inserted into the unit index), or over every unit counting only the amplitudes
+// with in-unit bit 0 clear (amp_mask = 1, complex64 lane 0).
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_prob_partial(const Cx<R> *__restrict__ a, BitInsert ins,
-                                                         u64 n_units, double *partial) {
+                                                         int amp_mask, u64 n_units, double *partial) {
     double s = 0.;
-    const u64 stride = (u64)gridDim.x * kBlock;
-    for (u64 u = (u64)blockIdx.x * kBlock + threadIdx.x; u < n_units; u += stride) {
-        const u64 i = insert_zero_bits(u * W, ins);
-        const Vec<R, W> v = *reinterpret_cast<const Vec<R, W> *>(a + i);
+    const u64 base = (u64)blockIdx.x * (kBlock * kProbUnitsPerThread) + threadIdx.x;
+#pragma unroll 4
+    for (int r = 0; r < kProbUnitsPerThread; ++r) {
+        const u64 w = base + (u64)r * kBlock;
+        if (w < n_units) {
+            const Amps<R> v = load_unit(a, insert_zero_bits(w, ins));
 #pragma unroll
-        for (int j = 0; j < W; ++j) s += abs2d(v.v[j]);
+            for (int j = 0; j < Unit<R>::W; ++j)
+                if ((j & amp_mask) == 0) s += abs2d(v.v[j]);
+        }
     }
-    s = block_sum(s);
+    s = block_sum<kBlock>(s);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *partial, int n, double *out) {
+__global__ __launch_bounds__(1024) void k_sum_partials(const double *partial, u64 n, double *out) {
     double s = 0.;
-    for (int i = threadIdx.x; i < n; i += kBlock) s += partial[i];
-    s = block_sum(s);
+    for (u64 i = threadIdx.x; i < n; i += 1024) s += partial[i];
+    s = block_sum<1024>(s);
     if (threadIdx.x == 0) *out = s;
 }
 
-template <class R, int W>
-__global__ __launch_bounds__(kBlock) void k_decohere(Cx<R> *__restrict__ a, BitInsert ins, u64 bit,
-                                                     int value, R norm, u64 n_units) {
-    const u64 u = (u64)blockIdx.x * kBlock + threadIdx.x;
-    if (u >= n_units) return;
-    typedef Vec<R, W> V;
-    const u64 i0 = insert_zero_bits(u * W, ins);
-    const u64 keep = value ? (i0 | bit) : i0, drop = value ? i0 : (i0 | bit);
-    V v = *reinterpret_cast<const V *>(a + keep);
-    V z;
+// Collapse in place. Lane at unit level (dist = unit distance of the lane bit): the kept unit
+// is scaled, its partner zeroed. Lane inside the unit (dist == 0): per amplitude.
+template <class R>
+__global__ __launch_bounds__(kWave) void k_decohere(Cx<R> *__restrict__ a, BitInsert ins, u64 dist,
+                                                    int value, R norm, u64 n_items, u64 first) {
+    const u64 w = first + (u64)blockIdx.x * kWave + threadIdx.x;
+    if (w >= n_items) return;
+    if (dist == 0) {   // complex64 lane 0: bit 0 of the amplitude inside the unit
+        Amps<R> v = load_unit(a, w);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+        for (int j = 0; j < Unit<R>::W; ++j)
+            v.v[j] = (j == value) ? cx<R>(v.v[j].re * norm, v.v[j].im * norm) : cx<R>(0, 0);
+        store_unit(a, w, v);
+        return;
+    }
+    const u64 u0 = insert_zero_bits(w, ins);
+    const u64 keep = value ? (u0 | dist) : u0, drop = value ? u0 : (u0 | dist);
+    Amps<R> v = load_unit(a, keep), z;
+#pragma unroll
+    for (int j = 0; j < Unit<R>::W; ++j) {
         v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
         z.v[j] = cx<R>(0, 0);
     }
-    *reinterpret_cast<V *>(a + keep) = v;
-    *reinterpret_cast<V *>(a + drop) = z;
+    store_unit(a, keep, v);
+    store_unit(a, drop, z);
 }
 
-template <class R, int W>
-__global__ __launch_bounds__(kBlock) void k_separate(Cx<R> *__restrict__ dst, const Cx<R> *__restrict__ src,
-                                                     BitInsert ins, u64 vbit, R norm, u64 n_units) {
-    const u64 u = (u64)blockIdx.x * kBlock + threadIdx.x;
+// dst[k] = norm * src[k with a `value` bit inserted at `lane`].
+template <class R>
+__global__ __launch_bounds__(kWave) void k_separate(Cx<R> *__restrict__ dst, const Cx<R> *__restrict__ src,
+                                                    BitInsert ins, u64 vbit, int value, R norm,
+                                                    u64 n_units, u64 first) {
+    const u64 u = first + (u64)blockIdx.x * kWave + threadIdx.x;
     if (u >= n_units) return;
-    typedef Vec<R, W> V;
-    const u64 i = insert_zero_bits(u * W, ins) | vbit;
-    V v = *reinterpret_cast<const V *>(src + i);
+    Amps<R> v;
+    bool gathered = false;
+    if constexpr (Unit<R>::W == 2) {
+        if (ins.n == 0) {   // complex64 lane 0: dst unit u takes amplitude `value` of src units 2u, 2u+1
+            const Amps<R> s0 = load_unit(src, 2 * u), s1 = load_unit(src, 2 * u + 1);
+            const Cx<R> a0 = s0.v[value], a1 = s1.v[value];
+            v.v[0] = cx<R>(norm * a0.re, norm * a0.im);
+            v.v[1] = cx<R>(norm * a1.re, norm * a1.im);
+            gathered = true;
+        }
+    }
+    if (!gathered) {
+        v = load_unit(src, insert_zero_bits(u, ins) | vbit);
 #pragma unroll
-    for (int j = 0; j < W; ++j) v.v[j] = cx<R>(norm * v.v[j].re, norm * v.v[j].im);
-    reinterpret_cast<V *>(dst)[u] = v;
+        for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(norm * v.v[j].re, norm * v.v[j].im);
+    }
+    store_unit(dst, u, v);
 }
 
-template <class R, int W>
-__global__ __launch_bounds__(kBlock) void k_reset(Cx<R> *__restrict__ a, BitInsert ins, u64 bit,
-                                                  u64 n_units) {
-    const u64 u = (u64)blockIdx.x * kBlock + threadIdx.x;
-    if (u >= n_units) return;
-    typedef Vec<R, W> V;
-    const u64 i0 = insert_zero_bits(u * W, ins);
-    const V v = *reinterpret_cast<const V *>(a + (i0 | bit));
-    V z;
+template <class R>
+__global__ __launch_bounds__(kWave) void k_reset(Cx<R> *__restrict__ a, BitInsert ins, u64 dist,
+                                                 u64 n_items, u64 first) {
+    const u64 w = first + (u64)blockIdx.x * kWave + threadIdx.x;
+    if (w >= n_items) return;
+    if (dist == 0) {   // complex64 lane 0
+        Amps<R> v = load_unit(a, w);
+        v.v[0] = v.v[Unit<R>::W - 1];
+        v.v[Unit<R>::W - 1] = cx<R>(0, 0);
+        store_unit(a, w, v);
+        return;
+    }
+    const u64 u0 = insert_zero_bits(w, ins);
+    const Amps<R> v = load_unit(a, u0 | dist);
+    Amps<R> z;
 #pragma unroll
-    for (int j = 0; j < W; ++j) z.v[j] = cx<R>(0, 0);
-    *reinterpret_cast<V *>(a + i0) = v;
-    *reinterpret_cast<V *>(a + (i0 | bit)) = z;
+    for (int j = 0; j < Unit<R>::W; ++j) z.v[j] = cx<R>(0, 0);
+    store_unit(a, u0, v);
+    store_unit(a, u0 | dist, z);
 }
 
 template <class R>
@@ -144,8 +181,8 @@ struct JoinArgs {
 // dst[i] = src[m-2][i_{m-2}] * src[m-1][i_{m-1}], then src[k][i_k] * (...) for k = m-3..0;
 // the same association as the reference's kron + in-place kron passes.
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_join(Cx<R> *__restrict__ dst, const JoinArgs<R> g) {
-    const u64 i = (u64)blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_join(Cx<R> *__restrict__ dst, const JoinArgs<R> g, u64 first) {
+    const u64 i = first + (u64)blockIdx.x * kBlock + threadIdx.x;
     if (i >= g.n_dst) return;
     if (i >= g.product) {
         dst[i] = cx<R>(0, 0);
@@ -162,24 +199,32 @@ __global__ __launch_bounds__(kBlock) void k_join(Cx<R> *__restrict__ dst, const 
     dst[i] = v;
 }
 
-BitInsert one_bit(int lane) {
+BitInsert one_bit(int pos) {
     BitInsert b;
     b.n = 1;
-    b.pos[0] = lane;
+    b.pos[0] = pos;
     return b;
 }
+
+BitInsert no_bits() {
+    BitInsert b;
+    b.n = 0;
+    return b;
+}
+
+constexpr int lw_of(int w) { return w == 2 ? 1 : 0; }
 
 }  // namespace
 
 template <class R>
 void launch_set_zero_state(Cx<R> *a, int n_lanes) {
-    const u64 n = u64(1) << n_lanes;
+    constexpr int W = Unit<R>::W;
     hipStream_t s = stream();
-    if (Unit<R>::W > 1 && n >= (u64)Unit<R>::W) {
-        constexpr int W = Unit<R>::W;
-        hipLaunchKernelGGL((k_zero_state<R, W>), blocks_for(n / W), kBlock, 0, s, a, n / W);
-    } else {
-        hipLaunchKernelGGL((k_zero_state<R, 1>), blocks_for(n), kBlock, 0, s, a, n);
+    if (n_lanes >= lw_of(W)) {
+        const u64 n_units = (u64(1) << n_lanes) / W;
+        for_each_piece(n_units, [&](u64 first, u64 count) {
+            hipLaunchKernelGGL((k_zero_state<R>), grid_for(count, kBlock), kBlock, 0, s, a, n_units, first);
+        });
     }
     QG_HIP(hipGetLastError());
 }
@@ -187,40 +232,51 @@ void launch_set_zero_state(Cx<R> *a, int n_lanes) {
 template <class R>
 double calc_prob(const Cx<R> *a, int n_lanes, int lane) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
     Runtime &rt = runtime();
     hipStream_t s = stream();
-    const BitInsert ins = one_bit(lane);
-    const u64 half = u64(1) << (n_lanes - 1);
-    constexpr int W = Unit<R>::W;
-    const bool vec = W > 1 && lane >= 1;
-    const u64 n_units = vec ? half / W : half;
-    const int grid = (int)std::min<u64>((u64)reduce_grid(), (n_units + kBlock - 1) / kBlock);
-    if (vec)
-        hipLaunchKernelGGL((k_prob_partial<R, W>), grid, kBlock, 0, s, a, ins, n_units, rt.d_partials);
-    else
-        hipLaunchKernelGGL((k_prob_partial<R, 1>), grid, kBlock, 0, s, a, ins, n_units, rt.d_partials);
-    QG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_partials, 1, kBlock, 0, s, rt.d_partials, grid,
-                       rt.d_partials + reduce_grid());
-    QG_HIP(hipGetLastError());
-    QG_HIP(hipMemcpyAsync(rt.h_pinned, rt.d_partials + reduce_grid(), sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-    QG_HIP(hipStreamSynchronize(s));
+    const u64 n_units_total = (u64(1) << n_lanes) / W;
+    BitInsert ins;
+    int amp_mask;
+    u64 n_units;
+    if (lane < lw) {   // complex64 lane 0: every unit, even amplitudes
+        ins = no_bits();
+        amp_mask = 1;
+        n_units = n_units_total;
+    } else {
+        ins = one_bit(lane - lw);
+        amp_mask = 0;
+        n_units = n_units_total / 2;
+    }
+    const u64 per_block = (u64)kBlock * kProbUnitsPerThread;
+    const u64 n_blocks = (n_units + per_block - 1) / per_block;
+    const size_t bytes = sizeof(double) * (size_t)(n_blocks + 1);
+    double *partial = static_cast<double *>(dev_alloc(bytes));
+    hipLaunchKernelGGL((k_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, ins, amp_mask, n_units,
+                       partial);
+    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, partial, n_blocks, partial + n_blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rt.h_pinned, partial + n_blocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(partial, bytes);
+    QG_HIP(e);
     return rt.h_pinned[0];
 }
 
 template <class R>
 void launch_decohere(Cx<R> *a, int n_lanes, int lane, int value, R norm) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
-    const u64 half = u64(1) << (n_lanes - 1);
-    constexpr int W = Unit<R>::W;
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
+    const u64 n_units = (u64(1) << n_lanes) / W;
     hipStream_t s = stream();
-    if (W > 1 && lane >= 1)
-        hipLaunchKernelGGL((k_decohere<R, W>), blocks_for(half / W), kBlock, 0, s, a, one_bit(lane),
-                           u64(1) << lane, value, norm, half / W);
-    else
-        hipLaunchKernelGGL((k_decohere<R, 1>), blocks_for(half), kBlock, 0, s, a, one_bit(lane),
-                           u64(1) << lane, value, norm, half);
+    const BitInsert ins = lane < lw ? no_bits() : one_bit(lane - lw);
+    const u64 dist = lane < lw ? 0 : u64(1) << (lane - lw);
+    const u64 n_items = lane < lw ? n_units : n_units / 2;
+    for_each_piece(n_items, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL((k_decohere<R>), grid_for(count, kWave), kWave, 0, s, a, ins, dist, value, norm,
+                           n_items, first);
+    });
     QG_HIP(hipGetLastError());
 }
 
@@ -228,31 +284,31 @@ template <class R>
 void launch_decohere_and_separate(Cx<R> *dst, const Cx<R> *src, int n_lanes, int lane, int value,
                                   R norm) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
-    const u64 half = u64(1) << (n_lanes - 1);
-    const u64 vbit = value ? (u64(1) << lane) : 0;
-    constexpr int W = Unit<R>::W;
+    QG_REQUIRE(n_lanes >= 2, "separating needs at least two lanes");
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
+    const u64 n_dst_units = (u64(1) << (n_lanes - 1)) / W;
     hipStream_t s = stream();
-    if (W > 1 && lane >= 1)
-        hipLaunchKernelGGL((k_separate<R, W>), blocks_for(half / W), kBlock, 0, s, dst, src,
-                           one_bit(lane), vbit, norm, half / W);
-    else
-        hipLaunchKernelGGL((k_separate<R, 1>), blocks_for(half), kBlock, 0, s, dst, src,
-                           one_bit(lane), vbit, norm, half);
+    const BitInsert ins = lane < lw ? no_bits() : one_bit(lane - lw);
+    const u64 vbit = (lane < lw || !value) ? 0 : (u64(1) << (lane - lw));
+    for_each_piece(n_dst_units, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL((k_separate<R>), grid_for(count, kWave), kWave, 0, s, dst, src, ins, vbit, value,
+                           norm, n_dst_units, first);
+    });
     QG_HIP(hipGetLastError());
 }
 
 template <class R>
 void launch_reset(Cx<R> *a, int n_lanes, int lane) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
-    const u64 half = u64(1) << (n_lanes - 1);
-    constexpr int W = Unit<R>::W;
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
+    const u64 n_units = (u64(1) << n_lanes) / W;
     hipStream_t s = stream();
-    if (W > 1 && lane >= 1)
-        hipLaunchKernelGGL((k_reset<R, W>), blocks_for(half / W), kBlock, 0, s, a, one_bit(lane),
-                           u64(1) << lane, half / W);
-    else
-        hipLaunchKernelGGL((k_reset<R, 1>), blocks_for(half), kBlock, 0, s, a, one_bit(lane),
-                           u64(1) << lane, half);
+    const BitInsert ins = lane < lw ? no_bits() : one_bit(lane - lw);
+    const u64 dist = lane < lw ? 0 : u64(1) << (lane - lw);
+    const u64 n_items = lane < lw ? n_units : n_units / 2;
+    for_each_piece(n_items, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL((k_reset<R>), grid_for(count, kWave), kWave, 0, s, a, ins, dist, n_items, first);
+    });
     QG_HIP(hipGetLastError());
 }
 
@@ -279,7 +335,9 @@ void launch_join(Cx<R> *dst, int dst_lanes, const std::vector<const Cx<R> *> &sr
     QG_REQUIRE(shift <= dst_lanes, "joined lanes exceed the destination");
     g.product = u64(1) << shift;
     g.n_dst = u64(1) << dst_lanes;
-    hipLaunchKernelGGL((k_join<R>), blocks_for(g.n_dst), kBlock, 0, stream(), dst, g);
+    for_each_piece(g.n_dst, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL((k_join<R>), grid_for(count, kBlock), kBlock, 0, stream(), dst, g, first);
+    });
     QG_HIP(hipGetLastError());
 }
 

@@ -25,6 +25,11 @@ Runtime &runtime() {
     return rt;
 }
 
+std::string &last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+
 namespace {
 
 constexpr size_t kCacheMaxBlock = size_t(1) << 30;   // cache blocks up to 1 GiB
@@ -50,6 +55,18 @@ void release_cache() {
 }
 
 }  // namespace
+
+namespace {
+u64 g_piece_items = u64(1) << 30;
+}
+
+u64 launch_piece_items() { return g_piece_items; }
+
+void set_launch_piece_items(u64 items) {
+    QG_REQUIRE(items >= (u64(1) << 12) && items <= (u64(1) << 30) && (items & (items - 1)) == 0,
+               "launch piece must be a power of two in [2^12, 2^30]");
+    g_piece_items = items;
+}
 
 void runtime_init(int device) {
     Runtime &rt = runtime();

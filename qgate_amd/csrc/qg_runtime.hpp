@@ -3,6 +3,7 @@
 #pragma once
 
 #include <mutex>
+#include <string>
 
 #include "qg_common.hpp"
 
@@ -21,6 +22,9 @@ struct Runtime {
 
 Runtime &runtime();
 
+// Message of the last failed C-ABI call on this thread.
+std::string &last_error();
+
 // Initialise the runtime on `device` (no-op if already initialised on it).
 void runtime_init(int device);
 // Release cached memory, scratch and the stream.
@@ -38,5 +42,21 @@ size_t dev_cached_bytes();
 
 // Grid size that fills the device for grid-stride reductions.
 inline int reduce_grid() { return 4096; }
+
+// A launch covers at most 2^32 - 1 work items, so kernels over index ranges that can exceed
+// that (one work item per amplitude or pair at 33+ qubits) are launched in pieces of
+// launch_piece_items() items (default 2^30; a power of two >= 2^12), each piece given the
+// index of its first item.
+u64 launch_piece_items();
+void set_launch_piece_items(u64 items);
+
+template <class F>
+void for_each_piece(u64 n_items, F &&f) {
+    const u64 piece = launch_piece_items();
+    for (u64 first = 0; first < n_items; first += piece)
+        f(first, n_items - first < piece ? n_items - first : piece);
+}
+
+inline unsigned grid_for(u64 count, int block) { return (unsigned)((count + block - 1) / block); }
 
 }  // namespace qg

@@ -55,6 +55,7 @@ def lib():
         'qghip_devices_initialize': (ctypes.c_int, [ctypes.c_int]),
         'qghip_devices_clear': (ctypes.c_int, []),
         'qghip_device_info': (ctypes.c_int, [_c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_uint64)]),
+        'qghip_set_launch_piece': (ctypes.c_int, [ctypes.c_int64]),
         'qghip_register_matrix_factory': (ctypes.c_int, [ctypes.c_char_p, _c_int_p]),
         'qghip_gate_matrix': (ctypes.c_int, [ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_int, _c_double_p]),
         'qghip_qubit_states_new': (ctypes.c_int, [ctypes.c_int, _c_void_pp]),
@@ -165,6 +166,11 @@ def device_info():
     dev, ncu, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
     check(lib().qghip_device_info(ctypes.byref(dev), ctypes.byref(ncu), ctypes.byref(mem)))
     return {'device_id': dev.value, 'n_cus': ncu.value, 'total_mem': mem.value}
+
+
+def set_launch_piece(items):
+    """Work items per kernel launch above which launches are split (power of two, 2^12..2^30)."""
+    check(lib().qghip_set_launch_piece(int(items)))
 
 
 # ---- gate matrices ---------------------------------------------------------------------

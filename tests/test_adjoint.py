@@ -1,0 +1,210 @@
+"""Adjoint differentiation: <H> and d<H>/dparams on the GPU against the CPU oracle.
+
+Bars: <H> and the final state within 1e-12 (FP64) / 5e-6 per amplitude (FP32) of the FP64
+oracle; gradients within 1e-10 (FP64) / 1e-4 (FP32) of the oracle's parameter-shift
+gradient (uncontrolled rotations) or within 1e-7 (FP64) of its central differences with
+h = 1e-5 (controlled rotations, truncation error O(h^2)). The checkers themselves are
+pinned by closed forms and by the dense Pauli matrix in the CPU tests.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from qgate_amd.adjoint import PauliSum
+from qgate_amd.model import gate_type as gtype
+from tests import adjoint_checks as ac
+
+X = np.array([[0, 1], [1, 0]], np.complex128)
+Y = np.array([[0, -1j], [1j, 0]], np.complex128)
+Z = np.diag([1., -1.]).astype(np.complex128)
+I2 = np.eye(2)
+
+
+# ---- CPU: the checkers ------------------------------------------------------------------
+
+def test_pauli_matrix_known_answers():
+    assert np.allclose(PauliSum([(1., {0: 'Y'})]).matrix(1), Y)
+    # qubit 1 is the high bit of the index: kron(op on qubit 1, op on qubit 0)
+    assert np.allclose(PauliSum([(2., {1: 'X'})]).matrix(2), 2. * np.kron(X, I2))
+    assert np.allclose(PauliSum([(1., {0: 'Z', 1: 'Y'})]).matrix(2), np.kron(Y, Z))
+    h = PauliSum.zz_x([0.5], [1., -1.])
+    ref = 0.5 * np.kron(Z, Z) + np.kron(I2, X) - np.kron(X, I2)
+    assert np.allclose(h.matrix(2), ref)
+
+
+def test_pauli_apply_matches_dense_matrix():
+    rs = np.random.RandomState(3)
+    n = 4
+    obs = ac.random_observable(n, 12, seed=5)
+    psi = rs.normal(size=1 << n) + 1j * rs.normal(size=1 << n)
+    assert np.allclose(ac.pauli_apply(obs, psi), obs.matrix(n) @ psi, atol=1e-13)
+
+
+@pytest.mark.parametrize('kind,obs,value,deriv', [
+    ('ry', {0: 'Z'}, math.cos, lambda t: -math.sin(t)),
+    ('ry', {0: 'X'}, math.sin, math.cos),
+    ('rx', {0: 'Z'}, math.cos, lambda t: -math.sin(t)),
+    ('rx', {0: 'Y'}, lambda t: -math.sin(t), lambda t: -math.cos(t)),
+])
+def test_checkers_closed_forms(port_module, kind, obs, value, deriv):
+    """One rotation on |0>: <Z> = cos t for RX/RY, <X> = sin t for RY, <Y> = -sin t for RX."""
+    spec = ac.CircuitSpec(1)
+    getattr(spec, kind)(0, 0)
+    h = PauliSum([(1., obs)])
+    for t in (0.3, 1.7, -2.2):
+        assert abs(ac.oracle_expectation(port_module, spec, [t], h) - value(t)) < 1e-14
+        g = ac.parameter_shift_gradient(port_module, spec, [t], h)
+        assert abs(g[0] - deriv(t)) < 1e-14
+        g = ac.central_difference_gradient(port_module, spec, [t], h)
+        assert abs(g[0] - deriv(t)) < 1e-9
+
+
+def test_checkers_agree_on_hea(port_module):
+    spec = ac.hea_spec(4, 2)
+    a, b = np.random.RandomState(1).normal(size=3), np.random.RandomState(2).normal(size=4)
+    h = PauliSum.zz_x(a, b)
+    params = np.random.RandomState(4).uniform(-math.pi, math.pi, spec.n_params)
+    ps = ac.parameter_shift_gradient(port_module, spec, params, h)
+    cd = ac.central_difference_gradient(port_module, spec, params, h)
+    assert np.max(np.abs(ps - cd)) < 1e-8
+    # a rotation left out of the shift (a dropped term) is caught by the comparison
+    assert np.max(np.abs(ps)) > 1e-3
+
+
+# ---- GPU ---------------------------------------------------------------------------------
+
+DTYPES = [np.float64, np.float32]
+GRAD_TOL = {np.float64: 1e-10, np.float32: 1e-4}
+VALUE_TOL = {np.float64: 1e-12, np.float32: 5e-5}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+def test_hea_expectation_and_state(hip_module, port_module, dtype):
+    n, layers = 10, 3
+    spec = ac.hea_spec(n, layers)
+    a, b = np.random.RandomState(8).normal(size=n - 1), np.random.RandomState(9).normal(size=n)
+    h = PauliSum.zz_x(a, b)
+    params = np.random.RandomState(10).uniform(-math.pi, math.pi, spec.n_params)
+    circ = spec.build(dtype)
+    value = circ.expectation(params, h)
+    ref_state = ac.oracle_state(port_module, spec, params)
+    ref = ac.expectation(h, ref_state)
+    print('value', value, 'ref', ref)
+    assert abs(value - ref) < VALUE_TOL[dtype] * max(1., abs(ref))
+    err = np.max(np.abs(circ.state() - ref_state))
+    assert err < (1e-12 if dtype == np.float64 else 5e-6), err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+def test_hea_gradient_vs_parameter_shift(hip_module, port_module, dtype):
+    n, layers = 6, 2
+    spec = ac.hea_spec(n, layers)
+    a, b = np.random.RandomState(18).normal(size=n - 1), np.random.RandomState(19).normal(size=n)
+    h = PauliSum.zz_x(a, b)
+    params = np.random.RandomState(20).uniform(-math.pi, math.pi, spec.n_params)
+    circ = spec.build(dtype)
+    value, grad = circ.gradient(params, h)
+    ref = ac.parameter_shift_gradient(port_module, spec, params, h)
+    err = np.max(np.abs(grad - ref))
+    print('grad err', err)
+    assert err < GRAD_TOL[dtype], err
+    assert abs(value - ac.oracle_expectation(port_module, spec, params, h)) < VALUE_TOL[dtype] * 10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('seed', [0, 1, 2])
+def test_mixed_gradient_vs_central_differences(hip_module, port_module, seed):
+    """Controlled rotations, fixed gates (adjoints too), shared parameters, X/Y/Z terms."""
+    n = 7
+    spec = ac.mixed_spec(n, 60, seed=seed, n_params=15)
+    h = ac.random_observable(n, 10, seed=seed + 50)
+    params = np.random.RandomState(seed + 70).uniform(-math.pi, math.pi, spec.n_params)
+    circ = spec.build(np.float64)
+    value, grad = circ.gradient(params, h)
+    ref_value = ac.oracle_expectation(port_module, spec, params, h)
+    assert abs(value - ref_value) < 1e-12
+    ref = ac.central_difference_gradient(port_module, spec, params, h)
+    err = np.max(np.abs(grad - ref))
+    print('seed', seed, 'grad err', err)
+    assert err < 1e-7, err
+
+
+@pytest.mark.gpu
+def test_backward_after_expectation_and_repeat(hip_module):
+    """backward reuses the forward state of the last expectation call; calling it again (or
+    with other parameters) recomputes the forward pass and gives the same gradient."""
+    spec = ac.mixed_spec(8, 40, seed=4)
+    h = ac.random_observable(8, 6, seed=44)
+    rs = np.random.RandomState(5)
+    p1, p2 = rs.uniform(-3, 3, spec.n_params), rs.uniform(-3, 3, spec.n_params)
+    circ = spec.build(np.float64)
+    v1, g1 = circ.gradient(p1, h)
+    assert circ.expectation(p1, h) == v1
+    assert np.array_equal(circ.backward(p1), g1)
+    assert np.array_equal(circ.backward(p1), g1)      # forward recomputed
+    _, g2 = circ.gradient(p2, h)
+    circ.expectation(p1, h)
+    assert np.array_equal(circ.backward(p2), g2)      # params differ: recomputed for p2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+def test_hea24_gradient_by_shift_on_device(hip_module, dtype):
+    """The 24-qubit, 12-layer ansatz (576 parameters): 16 seeded entries of the adjoint
+    gradient against parameter-shift differences of device expectations (exact for
+    uncontrolled rotations, so only rounding separates the two)."""
+    n, layers = 24, 12
+    spec = ac.hea_spec(n, layers)
+    a, b = np.random.RandomState(1004).normal(size=n - 1), np.random.RandomState(1005).normal(size=n)
+    h = PauliSum.zz_x(a, b)
+    params = np.random.RandomState(4).uniform(-math.pi, math.pi, spec.n_params)
+    circ = spec.build(dtype)
+    _, grad = circ.gradient(params, h)
+    tol = 1e-10 if dtype == np.float64 else 2e-3
+    subset = np.random.RandomState(44).choice(spec.n_params, 16, replace=False)
+    errs = []
+    for k in subset:
+        pp, pm = params.copy(), params.copy()
+        pp[k] += math.pi / 2
+        pm[k] -= math.pi / 2
+        ref = (circ.expectation(pp, h) - circ.expectation(pm, h)) / 2.
+        errs.append(abs(grad[k] - ref))
+    print('max err', max(errs))
+    assert max(errs) < tol, errs
+
+
+@pytest.mark.gpu
+def test_torch_autograd(hip_module):
+    import torch
+    from qgate_amd.adjoint import Expectation
+    spec = ac.hea_spec(6, 2)
+    h = PauliSum.zz_x(np.ones(5), np.full(6, 0.5))
+    circ = spec.build(np.float64)
+    params = np.random.RandomState(3).uniform(-1, 1, spec.n_params)
+    t = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    loss = 3. * Expectation.apply(t, circ, h)
+    loss.backward()
+    value, grad = circ.gradient(params, h)
+    assert abs(loss.item() - 3. * value) < 1e-13
+    assert np.allclose(t.grad.numpy(), 3. * grad, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_argument_errors(hip_module):
+    from qgate_amd.adjoint import ParametricCircuit
+    circ = ParametricCircuit(4, np.float64)
+    with pytest.raises(ValueError):
+        circ.ry(4, 0)                          # target out of range
+    with pytest.raises(ValueError):
+        circ.gate(gtype.X(), 1, controls=[1])  # control == target
+    circ.ry(0, 2)
+    with pytest.raises(ValueError):
+        circ.backward([0., 0., 0.])            # no observable yet
+    with pytest.raises(ValueError):
+        circ.expectation([0., 0.], PauliSum([(1., {0: 'Z'})]))   # too few parameters
+    with pytest.raises(ValueError):
+        circ.expectation([0., 0., 0.], PauliSum([(1., {5: 'Z'})]))  # term outside register
+    assert abs(circ.expectation([0., 0., math.pi / 3], PauliSum([(1., {0: 'Z'})])) - 0.5) < 1e-15
