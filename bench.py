@@ -15,6 +15,10 @@ roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, t
 HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
 launch time, against 8 TB/s.
 
+batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), each with its
+own parameters and observable, <H> and gradients in one graph replay, against a loop of
+single-circuit evaluations.
+
 adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qubit, 12-layer
 RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
 once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
@@ -131,6 +135,22 @@ def roofline(runner, n, amp, reps=4):
             'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
 
 
+def device_copy_ms(module, n):
+    """A plain copy kernel of known traffic for counter calibration: joining one 2^n-amplitude
+    complex64 vector into another (k_join with one source) reads and writes 8 * 2^n bytes."""
+    src = Runner(module, n, np.float32)
+    dst = module.create_qubit_states(np.float32)
+    dst.processor.initialize_qubit_states(dst, n)
+    src.sync()
+    t0 = time.perf_counter()
+    dst.processor.join(dst, [src.qs], 0)
+    dst.processor.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    dst.delete()
+    src.delete()
+    return {'bytes_each_way': 8 << n, 'ms': round(ms, 3)}
+
+
 def adjoint_timing(dtype, reps=3):
     """Forward (<H>) and backward (full gradient) times of the 24-qubit, 12-layer ansatz
     (576 parameters; H = sum a_i Z_i Z_i+1 + sum b_i X_i, a, b ~ N(0, 1), seed 4)."""
@@ -159,6 +179,47 @@ def adjoint_timing(dtype, reps=3):
             'gates': 3 * n * layers, 'params': 2 * n * layers}
 
 
+def batch_timing(n_circuits=4096, layers=10, seed=5, loop_sample=512):
+    """Batched adjoint: n_circuits ansatz circuits (RY, RZ on every qubit then a CZ ring per
+    layer) with qubit counts uniform in 8..20, each with its own parameters and ZZ + X
+    observable, evaluated by one graph replay; against a loop of single-circuit evaluations
+    (timed over the first loop_sample circuits and scaled to all)."""
+    from qgate_amd.adjoint import PauliSum, hea_circuit
+    from qgate_amd.batch import AdjointBatch
+    rs = np.random.RandomState(seed)
+    sizes = rs.randint(8, 21, n_circuits)
+    circuits = {int(n): hea_circuit(int(n), layers, np.float32) for n in sorted(set(sizes))}
+    batch = AdjointBatch(np.float32)
+    items = []
+    for n in sizes:
+        n = int(n)
+        params = rs.uniform(-math.pi, math.pi, 2 * n * layers)
+        h = PauliSum.zz_x(rs.normal(size=n - 1), rs.normal(size=n))
+        batch.add(circuits[n], params, h)
+        items.append((n, params, h))
+    t0 = time.perf_counter()
+    batch.run()                       # tunes, builds and captures the graph
+    first_s = time.perf_counter() - t0
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        values, grads = batch.run()
+    graph_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for n, params, h in items[:loop_sample]:
+        circuits[n].gradient(params, h)
+    loop_s = (time.perf_counter() - t0) * len(items) / loop_sample
+    tuning = batch.tuning
+    batch.delete()
+    for c in circuits.values():
+        c.delete()
+    return {'circuits': n_circuits, 'qubits': '8..20', 'layers': layers, 'dtype': 'complex64',
+            'graph_ms': round(graph_s * 1e3, 2), 'first_run_with_tuning_ms': round(first_s * 1e3, 1),
+            'loop_ms_scaled': round(loop_s * 1e3, 1), 'loop_sample': loop_sample,
+            'speedup_vs_loop': round(loop_s / graph_s, 2), 'circuits_per_s': round(n_circuits / graph_s, 1),
+            'groups': [{k: g[k] for k in ('qubits', 'items', 'tile_bits', 'threads', 'grid_per_cu')} for g in tuning]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -167,6 +228,9 @@ def main():
     ap.add_argument('--qubits', type=int, default=30)
     ap.add_argument('--gates', type=int, default=2000)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-batch', action='store_true')
+    ap.add_argument('--roofline-only', action='store_true',
+                    help='only the per-target gate timing loop and a 2 GiB copy kernel (for counter runs)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: the state-vector backend does not distribute qubits '
@@ -182,6 +246,12 @@ def main():
     info = hipruntime.device_info()
 
     r = Runner(hipruntime, n, np.float32)
+    if args.roofline_only:
+        roof = roofline(r, n, amp)
+        r.delete()
+        copy = device_copy_ms(hipruntime, 28)
+        print(json.dumps({'roofline': roof, 'copy': copy}))
+        return
     print('[bench] device {} ({} CUs), {} qubits complex64, {} gates, {:.1f} GB per step'
           .format(info['device_id'], info['n_cus'], n, len(gates), step_bytes / 1e9), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -195,6 +265,20 @@ def main():
     t1 = time.perf_counter()
     step_s = (t1 - t0) / args.steps
     value = step_bytes / step_s / 1e9
+
+    # the same step with gate fusion in tile passes (several gates per trip through HBM)
+    hipruntime.set_fusion('tile')
+    r.proc.set_fusion('tile')
+    r.apply(gates)
+    r.sync()
+    tf0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.apply(gates)
+        r.sync()
+    fused_s = (time.perf_counter() - tf0) / args.steps
+    hipruntime.set_fusion('off')
+    r.proc.set_fusion('off')
+    print('[bench] fused step {:.3f} s'.format(fused_s), file=sys.stderr, flush=True)
 
     # end-of-circuit work of the workload on the same state: pool + 2^20 draws, measures
     from qgate_amd.simulator.lanes import Lane
@@ -226,6 +310,8 @@ def main():
     for name, dt in (('complex64', np.float32), ('complex128', np.float64)):
         adjoint[name] = adjoint_timing(dt)
         print('[bench] adjoint 24q {}: {}'.format(name, adjoint[name]), file=sys.stderr, flush=True)
+    batched = batch_timing() if not args.no_batch else None
+    print('[bench] batch: {}'.format(batched), file=sys.stderr, flush=True)
 
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'GB/s', 'n_gpus': 1,
@@ -241,7 +327,11 @@ def main():
         'pool_build_ms': round((t3 - t2) * 1e3, 2),
         'pool_2p20_draws_ms': round((t4 - t3) * 1e3, 2),
         'measure_all_ms': round((t5 - t4) * 1e3, 2),
+        'fused_tile': {'ms_per_step': round(fused_s * 1e3, 2), 'gates_per_s': round(len(gates) / fused_s, 1),
+                       'gate_bytes_per_s_GBs': round(step_bytes / fused_s / 1e9, 1),
+                       'speedup_vs_unfused': round(step_s / fused_s, 2)},
         'adjoint_24q': adjoint,
+        'batched_adjoint': batched,
         'roofline': roof,
     }
     if not args.no_cpu:

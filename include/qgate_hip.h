@@ -32,6 +32,7 @@ extern "C" {
 enum { QGHIP_OK = 0, QGHIP_ERROR = -1, QGHIP_EVALUE = -2 };
 enum { QGHIP_FP64 = 0, QGHIP_FP32 = 1 };
 enum { QGHIP_MATHOP_NULL = 0, QGHIP_MATHOP_PROB = 1 };
+enum { QGHIP_FUSION_OFF = 0, QGHIP_FUSION_TILE = 1 };
 
 typedef struct qghip_qubit_states qghip_qubit_states;
 typedef struct qghip_qubit_processor qghip_qubit_processor;
@@ -79,6 +80,15 @@ int qghip_qubit_states_get_n_lanes(const qghip_qubit_states *qs, int *n_lanes);
 int qghip_qubit_processor_new(int prec, qghip_qubit_processor **out);
 /* qubit_processor_delete (glue.cpp:208-219). */
 int qghip_qubit_processor_delete(qghip_qubit_processor *p);
+/* Addition, no reference counterpart: gate fusion for the gates this processor applies.
+ * QGHIP_FUSION_TILE queues apply_gate / apply_controlled_gate / apply_matrix calls per state
+ * vector and applies them in tile passes (many gates per trip through HBM) when anything else
+ * touches that state vector, on synchronize, or every 4096 gates. tile_bits (<= 0: default
+ * 12 for FP32, 11 for FP64) bounds the tile to 2^tile_bits amplitudes; low_bits (< 0: default
+ * 5 / 4) is the number of lowest bits every tile keeps contiguous. QGHIP_FUSION_OFF applies
+ * each gate when called, as the reference does. */
+int qghip_qubit_processor_set_fusion(qghip_qubit_processor *p, int mode, int tile_bits,
+                                     int low_bits);
 /* qubit_processor_synchronize (glue.cpp:251-260): completes all queued device work. */
 int qghip_qubit_processor_synchronize(qghip_qubit_processor *p);
 /* qubit_processor_reset (glue.cpp:263-272). */
@@ -188,6 +198,10 @@ int qghip_circuit_add_gate(qghip_circuit *c, const double *mat8, const int *ctrl
  * G = X (pauli 1), Y (2) or Z (3): qgate's RX / RY / RZ. */
 int qghip_circuit_add_rotation(qghip_circuit *c, int pauli, int param_index, const int *ctrl,
                                int n_ctrl, int target);
+/* QGHIP_FUSION_TILE (default): forward and backward run in tile passes (many gates per trip
+ * through HBM; the backward sweep carries phi and lambda together); QGHIP_FUSION_OFF: one
+ * launch per gate and per backward step. */
+int qghip_circuit_set_fusion(qghip_circuit *c, int mode);
 int qghip_circuit_n_params(const qghip_circuit *c, int *n_params);
 /* <H> for H = sum_k coefs[k] P_k, P_k the Pauli string with X on xmask[k] bits, Z on
  * zmask[k] bits (both bits: Y). Leaves the final state and H|psi> on the device. */
@@ -203,6 +217,34 @@ int qghip_circuit_gradient(qghip_circuit *c, const double *params, int n_params,
                            double *value, double *grad);
 /* Copy of the final state of the last expectation call (2^n complex amplitudes). */
 int qghip_circuit_state(qghip_circuit *c, void *out, int64_t array_size);
+
+/* ---- batches of parameterised circuits (no reference counterpart) --------------------- */
+
+/* Many circuits of uneven qubit counts, each with its own parameters and Pauli-sum
+ * coefficients, evaluated together: <H> and the full adjoint gradient of every item in one
+ * captured HIP graph of persistent tile-pass kernels. Items added with the same circuit
+ * handle, qubit count and observable terms (masks; the coefficients may differ) share a
+ * group. The first run tunes tile size, workgroup size and grid size per group by timing
+ * candidates, then captures the graph; later runs replay it with the current parameters.
+ * Callers: variational workloads that evaluate many small circuits at once. */
+typedef struct qghip_batch qghip_batch;
+int qghip_batch_new(int prec, qghip_batch **out);
+int qghip_batch_delete(qghip_batch *b);
+/* Append an item: the gate structure of circuit `c` (copied; c must stay alive while items
+ * built from it are added), its parameters and observable. *item receives its index. */
+int qghip_batch_add(qghip_batch *b, const qghip_circuit *c, const double *params, int n_params, int n_terms,
+                    const double *coefs, const uint64_t *xmask, const uint64_t *zmask, int *item);
+/* Replace the parameters of one item (used by the next run). */
+int qghip_batch_set_params(qghip_batch *b, int item, const double *params, int n_params);
+/* Fix tuning knobs instead of searching them (0: search): tile bits (1..12), threads per
+ * workgroup (128 or 256), persistent workgroups per compute unit. Re-tunes on the next run. */
+int qghip_batch_set_tuning(qghip_batch *b, int tile_bits, int threads, int grid_per_cu);
+/* Evaluate every item: values[i] = <H>_i; grads (if not null) receives each item's gradient
+ * (its circuit's n_params entries) in item order. use_graph = 0 launches the same kernels
+ * without the graph. */
+int qghip_batch_run(qghip_batch *b, int use_graph, double *values, double *grads, int64_t grads_size);
+/* JSON list of the groups with their chosen tuning. */
+int qghip_batch_describe(const qghip_batch *b, char *buf, int size);
 
 /* ---- launch shape (no reference counterpart) ----------------------------------------- */
 

@@ -38,6 +38,7 @@ def _lib():
             'qghip_circuit_add_gate': [H, P_D, P_I, ctypes.c_int, ctypes.c_int],
             'qghip_circuit_add_rotation': [H, ctypes.c_int, ctypes.c_int, P_I, ctypes.c_int, ctypes.c_int],
             'qghip_circuit_n_params': [H, P_I],
+            'qghip_circuit_set_fusion': [H, ctypes.c_int],
             'qghip_circuit_expectation': [H, P_D, ctypes.c_int, ctypes.c_int, P_D, P_U, P_U, P_D],
             'qghip_circuit_backward': [H, P_D, ctypes.c_int, P_D],
             'qghip_circuit_gradient': [H, P_D, ctypes.c_int, ctypes.c_int, P_D, P_U, P_U, P_D, P_D],
@@ -130,6 +131,15 @@ class ParametricCircuit:
         if getattr(self, 'ptr', None) is not None and hipruntime._lib is not None:
             hipruntime._lib.qghip_circuit_delete(self.ptr)
         self.ptr = None
+
+    def set_fusion(self, mode):
+        """'tile' (default): tile passes for the forward pass and the backward sweep; 'off': one
+        launch per gate and per backward step."""
+        modes = {'off': 0, 'tile': 1}
+        if mode not in modes:
+            raise ValueError('fusion mode must be one of {}'.format(sorted(modes)))
+        hipruntime.check(_lib().qghip_circuit_set_fusion(self.ptr, modes[mode]))
+        return self
 
     def _rot(self, pauli, qubit, param, controls):
         ctrl = list(controls)

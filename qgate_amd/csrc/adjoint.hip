@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/qgate_hip.h"
+#include "circuit.hpp"
 #include "gate_matrix.hpp"
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
@@ -32,14 +33,7 @@ constexpr int kBlock = 256;
 
 inline unsigned blocks_for(u64 n, int b) { return (unsigned)((n + b - 1) / b); }
 
-struct Op {
-    int kind;              // 0 fixed matrix, 1 rotation exp(-i theta G / 2)
-    int pauli;             // 1 X, 2 Y, 3 Z (rotations)
-    int param;             // parameter index (rotations): theta = params[param]
-    GateMatrix mat;        // fixed gates
-    std::vector<int> ctrl;
-    int target;
-};
+typedef CircuitOp Op;
 
 GateMatrix rotation(int pauli, double theta) {
     GateMatrix m;
@@ -207,20 +201,6 @@ __global__ __launch_bounds__(kBlock) void k_backward_step(Cx<R> *__restrict__ ph
 
 }  // namespace
 
-struct qghip_circuit {
-    Prec prec;
-    int n_qubits;
-    std::vector<Op> ops;
-    int n_params = 0;
-    void *psi = nullptr, *lam = nullptr;   // device state vectors
-    size_t state_bytes = 0;
-    // state left by the last expectation call: psi = final state, lam = H psi
-    bool forward_valid = false;
-    std::vector<double> forward_params;
-    std::vector<double> obs_coefs;
-    std::vector<uint64_t> obs_xmask, obs_zmask;
-};
-
 namespace {
 
 template <class F>
@@ -295,7 +275,21 @@ double forward_impl(qghip_circuit *c, const double *params, const Observable &ob
     Cx<R> *psi = static_cast<Cx<R> *>(c->psi), *lam = static_cast<Cx<R> *>(c->lam);
     hipStream_t s = stream();
     launch_set_zero_state<R>(psi, c->n_qubits);
-    for (const Op &op : c->ops) launch_gate<R>(psi, c->n_qubits, op_matrix(op, params), op.ctrl, op.target);
+    if (c->fusion == QGHIP_FUSION_TILE) {
+        std::vector<QueuedGate> gates;
+        gates.reserve(c->ops.size());
+        for (const Op &op : c->ops) gates.push_back(QueuedGate{op_matrix(op, params), op.ctrl, op.target});
+        apply_gate_sequence<R>(psi, c->n_qubits, gates, 0, -1);
+    } else {
+        for (const Op &op : c->ops) launch_gate<R>(psi, c->n_qubits, op_matrix(op, params), op.ctrl, op.target);
+    }
+    if (c->fusion == QGHIP_FUSION_TILE) {
+        std::vector<PauliGroup> groups;
+        std::vector<PauliTerm> terms;
+        for (const GroupDev &g : obs.groups) groups.push_back(PauliGroup{g.xmask, g.first, g.count, 0u});
+        for (const TermDev &t : obs.terms) terms.push_back(PauliTerm{t.cre, t.cim, t.zmask});
+        return pauli_expectation<R>(psi, lam, c->n_qubits, groups, terms);
+    }
     const u64 n = u64(1) << c->n_qubits;
     const size_t obs_bytes = sizeof(GroupDev) * obs.groups.size() + sizeof(TermDev) * obs.terms.size();
     char *d_obs = static_cast<char *>(dev_alloc(obs_bytes + 16));
@@ -329,7 +323,42 @@ double forward_impl(qghip_circuit *c, const double *params, const Observable &ob
 }
 
 template <class R>
+void backward_tiles(qghip_circuit *c, const double *params, double *grad) {
+    Cx<R> *phi = static_cast<Cx<R> *>(c->psi), *lam = static_cast<Cx<R> *>(c->lam);
+    const int n_ops = (int)c->ops.size();
+    std::vector<SweepGate> gates;
+    gates.reserve(n_ops);
+    for (int k = n_ops - 1; k >= 0; --k) {   // the sweep runs from the last gate back
+        const Op &op = c->ops[k];
+        gates.push_back(SweepGate{dagger(op_matrix(op, params)), op.kind == 1 ? op.pauli : 0, op.ctrl, op.target});
+    }
+    const size_t bytes = sizeof(double) * (size_t)std::max(1, n_ops);
+    double *d_terms = static_cast<double *>(dev_alloc(bytes));
+    std::vector<double> terms(std::max(1, n_ops));
+    hipError_t e = hipSuccess;
+    try {
+        adjoint_sweep<R>(phi, lam, c->n_qubits, gates, d_terms, 0, -1);
+        e = hipMemcpyAsync(terms.data(), d_terms, bytes, hipMemcpyDeviceToHost, stream());
+        if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    } catch (...) {
+        dev_free(d_terms, bytes);
+        throw;
+    }
+    dev_free(d_terms, bytes);
+    QG_HIP(e);
+    for (int i = 0; i < c->n_params; ++i) grad[i] = 0.;
+    for (int i = 0; i < n_ops; ++i) {   // sweep order: last gate first
+        const Op &op = c->ops[n_ops - 1 - i];
+        if (op.kind == 1) grad[op.param] += terms[i];
+    }
+}
+
+template <class R>
 void backward_impl(qghip_circuit *c, const double *params, double *grad) {
+    if (c->fusion == QGHIP_FUSION_TILE) {
+        backward_tiles<R>(c, params, grad);
+        return;
+    }
     Cx<R> *phi = static_cast<Cx<R> *>(c->psi), *lam = static_cast<Cx<R> *>(c->lam);
     hipStream_t s = stream();
     const int n = c->n_qubits;
@@ -437,6 +466,15 @@ int qghip_circuit_add_rotation(qghip_circuit *c, int pauli, int param_index, con
         op.target = target;
         c->ops.push_back(op);
         c->n_params = std::max(c->n_params, param_index + 1);
+    });
+}
+
+int qghip_circuit_set_fusion(qghip_circuit *c, int mode) {
+    return guarded_adj([&] {
+        require_arg(c != nullptr, "null argument");
+        require_arg(mode == QGHIP_FUSION_OFF || mode == QGHIP_FUSION_TILE, "unknown fusion mode");
+        c->fusion = mode;
+        c->forward_valid = false;
     });
 }
 

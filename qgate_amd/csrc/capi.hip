@@ -17,12 +17,19 @@ struct qghip_qubit_states {
     int n_lanes = -1;
     void *d_ptr = nullptr;
     size_t bytes = 0;
+    // gates queued by a processor with fusion on, applied in tile passes before anything
+    // else reads or writes this state vector
+    std::vector<QueuedGate> pending;
+    int tile_bits = 0, low_bits = -1;
 
     size_t n_states() const { return size_t(1) << n_lanes; }
 };
 
 struct qghip_qubit_processor {
     Prec prec;
+    int fusion = 0;             // QGHIP_FUSION_OFF or QGHIP_FUSION_TILE
+    int tile_bits = 0;          // <= 0: default for the precision
+    int low_bits = -1;          // < 0: default
 };
 
 struct qghip_qubits_states_getter {
@@ -76,6 +83,46 @@ void check_lane(const qghip_qubit_states *qs, int lane) {
     if (lane < 0 || lane >= qs->n_lanes) value_error("lane out of range");
 }
 
+// State vectors with queued gates (so that synchronize can apply them all).
+std::vector<qghip_qubit_states *> &pending_states() {
+    static std::vector<qghip_qubit_states *> v;
+    return v;
+}
+
+void unlist(qghip_qubit_states *qs) {
+    auto &v = pending_states();
+    v.erase(std::remove(v.begin(), v.end(), qs), v.end());
+}
+
+// Apply the queued gates of qs.
+void flush(qghip_qubit_states *qs) {
+    if (qs == nullptr || qs->pending.empty()) return;
+    std::vector<QueuedGate> gates;
+    gates.swap(qs->pending);
+    unlist(qs);
+    if (qs->d_ptr == nullptr) return;
+    if (qs->prec == precFP32)
+        apply_gate_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
+                                   qs->low_bits);
+    else
+        apply_gate_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
+                                    qs->low_bits);
+}
+
+// Forget the queued gates of qs (its contents are about to be overwritten or freed).
+void drop(qghip_qubit_states *qs) {
+    if (qs == nullptr || qs->pending.empty()) return;
+    qs->pending.clear();
+    unlist(qs);
+}
+
+void flush_all() {
+    std::vector<qghip_qubit_states *> v(pending_states());
+    for (qghip_qubit_states *qs : v) flush(qs);
+}
+
+constexpr size_t kMaxQueued = 4096;
+
 void release(qghip_qubit_states *qs) {
     if (qs->d_ptr) dev_free(qs->d_ptr, qs->bytes);
     qs->d_ptr = nullptr;
@@ -101,8 +148,8 @@ void by_prec(Prec p, F &&f) {
         f(double());
 }
 
-void apply_matrix(Prec prec, qghip_qubit_states *qs, const GateMatrix &m, const int *ctrl,
-                  int n_ctrl, int target) {
+void apply_matrix(const qghip_qubit_processor *p, Prec prec, qghip_qubit_states *qs,
+                  const GateMatrix &m, const int *ctrl, int n_ctrl, int target) {
     check_qs(qs, prec);
     check_lane(qs, target);
     if (n_ctrl < 0 || (n_ctrl > 0 && ctrl == nullptr)) value_error("bad control lane list");
@@ -115,6 +162,15 @@ void apply_matrix(Prec prec, qghip_qubit_states *qs, const GateMatrix &m, const 
     std::sort(sorted.begin(), sorted.end());
     if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
         value_error("duplicate control lanes");
+    if (p != nullptr && p->fusion == QGHIP_FUSION_TILE) {
+        if (qs->pending.empty()) pending_states().push_back(qs);
+        qs->tile_bits = p->tile_bits;
+        qs->low_bits = p->low_bits;
+        qs->pending.push_back(QueuedGate{m, controls, target});
+        if (qs->pending.size() >= kMaxQueued) flush(qs);
+        return;
+    }
+    flush(qs);
     by_prec(prec, [&](auto r) {
         typedef decltype(r) R;
         launch_gate<R>(amps<R>(qs), qs->n_lanes, m, controls, target);
@@ -129,6 +185,7 @@ std::vector<StateView> make_views(const int *lane_tables, qghip_qubit_states *co
     int off = 0;
     for (int q = 0; q < n; ++q) {
         check_qs(list[q], prec);
+        flush(list[q]);
         StateView v;
         v.ptr = list[q]->d_ptr;
         v.n_lanes = list[q]->n_lanes;
@@ -200,6 +257,7 @@ int qghip_qubit_states_new(int prec, qghip_qubit_states **out) {
 int qghip_qubit_states_delete(qghip_qubit_states *qs) {
     return guarded([&] {
         if (qs == nullptr) return;
+        drop(qs);
         release(qs);
         delete qs;
     });
@@ -208,6 +266,7 @@ int qghip_qubit_states_delete(qghip_qubit_states *qs) {
 int qghip_qubit_states_deallocate(qghip_qubit_states *qs) {
     return guarded([&] {
         if (qs == nullptr) value_error("null qubit states handle");
+        drop(qs);
         release(qs);
     });
 }
@@ -232,9 +291,22 @@ int qghip_qubit_processor_delete(qghip_qubit_processor *p) {
     return guarded([&] { delete p; });
 }
 
+int qghip_qubit_processor_set_fusion(qghip_qubit_processor *p, int mode, int tile_bits, int low_bits) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        if (mode != QGHIP_FUSION_OFF && mode != QGHIP_FUSION_TILE) value_error("unknown fusion mode");
+        if (tile_bits > 14) value_error("tile_bits out of range");
+        if (low_bits > 8) value_error("low_bits out of range");
+        p->fusion = mode;
+        p->tile_bits = tile_bits;
+        p->low_bits = low_bits;
+    });
+}
+
 int qghip_qubit_processor_synchronize(qghip_qubit_processor *p) {
     return guarded([&] {
         if (p == nullptr) value_error("null processor handle");
+        flush_all();
         sync();
     });
 }
@@ -251,6 +323,7 @@ int qghip_qubit_processor_initialize_qubit_states(qghip_qubit_processor *p, qghi
         if (p == nullptr || qs == nullptr) value_error("null argument");
         if (qs->prec != p->prec) value_error("qubit states precision does not match");
         if (n_lanes < 1 || n_lanes > 40) value_error("n_lanes out of range (1..40)");
+        drop(qs);
         release(qs);
         qs->n_lanes = n_lanes;
         qs->bytes = amp_bytes(qs->prec) << n_lanes;
@@ -262,6 +335,7 @@ int qghip_qubit_processor_reset_qubit_states(qghip_qubit_processor *p, qghip_qub
     return guarded([&] {
         if (p == nullptr) value_error("null processor handle");
         check_qs(qs, p->prec);
+        drop(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             launch_set_zero_state<R>(amps<R>(qs), qs->n_lanes);
@@ -275,6 +349,7 @@ int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit
         if (p == nullptr || prob == nullptr) value_error("null argument");
         check_qs(qs, p->prec);
         check_lane(qs, local_lane);
+        flush(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             *prob = calc_prob<R>(camps<R>(qs), qs->n_lanes, local_lane);
@@ -298,6 +373,8 @@ int qghip_qubit_processor_join(qghip_qubit_processor *p, qghip_qubit_states *qs,
             total += qstates_list[i]->n_lanes;
         }
         if (total + n_new_lanes != qs->n_lanes) value_error("joined lane count mismatch");
+        for (int i = 0; i < n_qstates; ++i) flush(qstates_list[i]);
+        drop(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             std::vector<const Cx<R> *> srcs;
@@ -314,6 +391,7 @@ int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double p
         check_qs(qs, p->prec);
         check_lane(qs, local_lane);
         if (value != 0 && value != 1) value_error("value must be 0 or 1");
+        flush(qs);
         // norm in FP64, rounded to the state precision (CPUQubitProcessor.cpp:227, :235)
         const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
         by_prec(p->prec, [&](auto r) {
@@ -335,6 +413,9 @@ int qghip_qubit_processor_decohere_and_separate(qghip_qubit_processor *p, int va
         if (value != 0 && value != 1) value_error("value must be 0 or 1");
         if (qs0->n_lanes != qs->n_lanes - 1 || qs1->n_lanes != 1)
             value_error("separated qubit states have wrong sizes");
+        flush(qs);
+        drop(qs0);
+        drop(qs1);
         const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
@@ -351,6 +432,7 @@ int qghip_qubit_processor_apply_reset(qghip_qubit_processor *p, qghip_qubit_stat
         if (p == nullptr) value_error("null processor handle");
         check_qs(qs, p->prec);
         check_lane(qs, local_lane);
+        flush(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             launch_reset<R>(amps<R>(qs), qs->n_lanes, local_lane);
@@ -364,7 +446,7 @@ int qghip_qubit_processor_apply_gate(qghip_qubit_processor *p, int factory_id, c
     return guarded([&] {
         if (p == nullptr) value_error("null processor handle");
         GateMatrix m = make_gate_matrix(factory_id, args, n_args, adjoint != 0);
-        apply_matrix(p->prec, qs, m, nullptr, 0, local_lane);
+        apply_matrix(p, p->prec, qs, m, nullptr, 0, local_lane);
     });
 }
 
@@ -376,7 +458,7 @@ int qghip_qubit_processor_apply_controlled_gate(qghip_qubit_processor *p, int fa
     return guarded([&] {
         if (p == nullptr) value_error("null processor handle");
         GateMatrix m = make_gate_matrix(factory_id, args, n_args, adjoint != 0);
-        apply_matrix(p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
+        apply_matrix(p, p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
     });
 }
 
@@ -385,7 +467,7 @@ int qghip_qubit_processor_apply_matrix(qghip_qubit_processor *p, const double *m
                                        int n_controls, int local_target_lane) {
     return guarded([&] {
         if (p == nullptr || mat8 == nullptr) value_error("null argument");
-        apply_matrix(p->prec, qs, gate_matrix_from_array(mat8), local_control_lanes, n_controls,
+        apply_matrix(p, p->prec, qs, gate_matrix_from_array(mat8), local_control_lanes, n_controls,
                      local_target_lane);
     });
 }
@@ -398,6 +480,7 @@ int qghip_qubit_processor_apply_unitary(qghip_qubit_processor *p, qghip_qubit_st
         if (k < 1 || k > 6 || k > qs->n_lanes) value_error("unitary width out of range");
         std::vector<int> ls(lanes, lanes + k);
         for (int l : ls) check_lane(qs, l);
+        flush(qs);
         const int D = 1 << k;
         std::vector<std::complex<double>> m(D * D);
         for (int i = 0; i < D * D; ++i) m[i] = std::complex<double>(mat[2 * i], mat[2 * i + 1]);
@@ -557,13 +640,14 @@ int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const d
         if (p == nullptr || mat8 == nullptr || avg_ms == nullptr || reps < 1)
             value_error("bad argument");
         GateMatrix m = gate_matrix_from_array(mat8);
+        flush(qs);
         hipStream_t s = stream();
         hipEvent_t e0, e1;
         QG_HIP(hipEventCreate(&e0));
         QG_HIP(hipEventCreate(&e1));
         QG_HIP(hipEventRecord(e0, s));
         for (int i = 0; i < reps; ++i)
-            apply_matrix(p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
+            apply_matrix(nullptr, p->prec, qs, m, local_control_lanes, n_controls, local_target_lane);
         QG_HIP(hipEventRecord(e1, s));
         QG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -1,0 +1,688 @@
+// Tile passes: many gates per trip of the state vector through HBM.
+//
+// A pass picks a set T of m state bits (m <= 12 for complex64, 11 for complex128) that always
+// holds the lowest `low` bits (so each tile row is a contiguous run of 2^low amplitudes) plus
+// the targets of the gates it carries. The state splits into 2^(n-m) tiles of 2^m amplitudes
+// (the bits outside T fixed per tile); each workgroup loads one tile into LDS, applies the
+// pass's gates there in order and writes the tile back. Per gate:
+//   * the target is a tile bit (or the gate is diagonal: a target outside the tile only picks
+//     m00 or m11 for the whole tile);
+//   * controls inside the tile are checked per amplitude; controls outside it are one
+//     predicate per tile, so they never use a tile bit.
+// Gates are applied in program order with the reference's pair update
+// (CPUQubitProcessor.cpp:307-362): a[i0] <- m00 a[i0] + m01 a[i1], a[i1] <- m10 a[i0] + m11 a[i1].
+// The gate list of a pass travels in the kernel arguments (at most 40 / 24 gates).
+#include <algorithm>
+#include <vector>
+
+#include "kernels.hpp"
+#include "qg_runtime.hpp"
+#include "tile_common.hpp"
+
+namespace qg {
+
+namespace {
+
+constexpr int kTileThreads = 256;
+
+template <class R>
+struct TileCfg;
+template <>
+struct TileCfg<float> {
+    static constexpr int MaxBits = 12;   // 2^12 x 8 B = 32 KiB of LDS
+    static constexpr int MaxOps = 40;
+    static constexpr int Low = 5;        // 32 contiguous amplitudes = 256 B per row
+};
+template <>
+struct TileCfg<double> {
+    static constexpr int MaxBits = 11;   // 2^11 x 16 B = 32 KiB
+    static constexpr int MaxOps = 24;
+    static constexpr int Low = 4;        // 16 x 16 B = 256 B per row
+};
+
+template <class R>
+struct TileOp {
+    Mat2<R> m;
+    u64 gctrl;          // control bits outside the tile (state positions)
+    unsigned lctrl;     // control bits inside the tile (local positions)
+    short target;       // local bit (pair, diag-local) or state bit (diag-outer)
+    short kind;
+};
+
+template <class R>
+struct TileProgram {
+    int m;              // tile bits
+    int low;            // pos[j] == j for j < low
+    int n_ops;
+    signed char pos[16];   // state bit of local bit j, ascending
+    TileOp<R> ops[TileCfg<R>::MaxOps];
+};
+
+template <class R>
+__device__ __forceinline__ void tile_mix(Cx<R> *t, const Mat2<R> &mm, int i0, int i1) {
+    const Cx<R> x0 = t[i0], x1 = t[i1];
+    t[i0] = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
+    t[i1] = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
+}
+
+template <class R, int MB>
+__global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ a, const TileProgram<R> prog,
+                                                            u64 first) {
+    __shared__ Cx<R> t[1 << MB];
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const int m = prog.m;
+    const u64 base = tile_base(first + blockIdx.x, prog.pos, m);
+    const int n_units = 1 << (m - lw);
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        const Amps<R> v = load_unit(a, (base | deposit(l, prog.pos, m, prog.low)) >> lw);
+#pragma unroll
+        for (int j = 0; j < W; ++j) t[l + j] = v.v[j];
+    }
+    __syncthreads();
+    const int n_amps = 1 << m;
+    for (int k = 0; k < prog.n_ops; ++k) {
+        const TileOp<R> &op = prog.ops[k];
+        if ((base & op.gctrl) != op.gctrl) continue;   // same for the whole workgroup
+        const unsigned lc = op.lctrl;
+        if (op.kind == kOpPair) {
+            const int tb = op.target;
+            const unsigned lowmask = (1u << tb) - 1;
+            for (int p = threadIdx.x; p < n_amps / 2; p += kTileThreads) {
+                const unsigned i0 = (((unsigned)p & ~lowmask) << 1) | ((unsigned)p & lowmask);
+                if ((i0 & lc) == lc) tile_mix(t, op.m, (int)i0, (int)(i0 | (1u << tb)));
+            }
+        }
+        if (op.kind == kOpDiagLocal) {
+            const int tb = op.target;
+            for (int i = threadIdx.x; i < n_amps; i += kTileThreads) {
+                if (((unsigned)i & lc) == lc) t[i] = cmul(((i >> tb) & 1) ? op.m.m11 : op.m.m00, t[i]);
+            }
+        }
+        if (op.kind == kOpDiagOuter) {
+            const Cx<R> f = ((base >> op.target) & 1) ? op.m.m11 : op.m.m00;
+            for (int i = threadIdx.x; i < n_amps; i += kTileThreads) {
+                if (((unsigned)i & lc) == lc) t[i] = cmul(f, t[i]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        Amps<R> v;
+#pragma unroll
+        for (int j = 0; j < W; ++j) v.v[j] = t[l + j];
+        store_unit(a, (base | deposit(l, prog.pos, m, prog.low)) >> lw, v);
+    }
+}
+
+bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
+
+// Pauli X (1), Y (2), Z (3); zero matrix otherwise.
+void set_pauli(GateMatrix &g, int pauli) {
+    const std::complex<double> J(0., 1.);
+    g.m[0][0] = g.m[0][1] = g.m[1][0] = g.m[1][1] = 0.;
+    if (pauli == 1) {
+        g.m[0][1] = 1.;
+        g.m[1][0] = 1.;
+    } else if (pauli == 2) {
+        g.m[0][1] = -J;
+        g.m[1][0] = J;
+    } else if (pauli == 3) {
+        g.m[0][0] = 1.;
+        g.m[1][1] = -1.;
+    }
+}
+
+// One pass being built: the tile bits and the gates it carries.
+struct PassPlan {
+    std::vector<int> bits;                 // state bits of the tile (unsorted while building)
+    std::vector<const QueuedGate *> gates;
+    bool has(int b) const { return std::find(bits.begin(), bits.end(), b) != bits.end(); }
+};
+
+template <class R>
+void launch_pass(Cx<R> *a, int n_lanes, const PassPlan &plan, int low, int max_bits) {
+    typedef TileCfg<R> C;
+    if (plan.gates.size() == 1) {   // a lone gate streams faster than through LDS
+        const QueuedGate &g = *plan.gates[0];
+        launch_gate<R>(a, n_lanes, g.m, g.ctrl, g.target);
+        return;
+    }
+    TileProgram<R> prog;
+    std::vector<int> bits(plan.bits);
+    // fill the tile up to its full size with the lowest free bits: bigger tiles mean fewer,
+    // fuller workgroups and longer contiguous rows
+    for (int b = 0; (int)bits.size() < max_bits && b < n_lanes; ++b)
+        if (std::find(bits.begin(), bits.end(), b) == bits.end()) bits.push_back(b);
+    std::sort(bits.begin(), bits.end());
+    prog.m = (int)bits.size();
+    prog.low = low;
+    for (int j = 0; j < 16; ++j) prog.pos[j] = j < prog.m ? (signed char)bits[j] : (signed char)0;
+    int local[64];
+    for (int b = 0; b < 64; ++b) local[b] = -1;
+    for (int j = 0; j < prog.m; ++j) local[bits[j]] = j;
+    prog.n_ops = (int)plan.gates.size();
+    QG_REQUIRE(prog.n_ops <= C::MaxOps, "tile pass holds too many gates");
+    for (int k = 0; k < prog.n_ops; ++k) {
+        const QueuedGate &g = *plan.gates[k];
+        TileOp<R> &op = prog.ops[k];
+        op.m = round_matrix<R>(g.m);
+        op.gctrl = 0;
+        op.lctrl = 0;
+        for (int c : g.ctrl) {
+            if (local[c] >= 0)
+                op.lctrl |= 1u << local[c];
+            else
+                op.gctrl |= u64(1) << c;
+        }
+        if (local[g.target] >= 0) {
+            op.target = (short)local[g.target];
+            op.kind = (short)(is_diagonal(g.m) ? kOpDiagLocal : kOpPair);
+        } else {
+            QG_REQUIRE(is_diagonal(g.m), "tile pass: non-diagonal gate outside the tile");
+            op.target = (short)g.target;
+            op.kind = (short)kOpDiagOuter;
+        }
+    }
+    const u64 n_tiles = u64(1) << (n_lanes - prog.m);
+    hipStream_t s = stream();
+    // one workgroup per tile; pieces keep every grid within launch_piece_items() threads
+    const u64 piece = launch_piece_items() / kTileThreads;
+    for (u64 first = 0; first < n_tiles; first += piece) {
+        const u64 count = std::min(piece, n_tiles - first);
+        hipLaunchKernelGGL((k_tile_pass<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, a, prog, first);
+    }
+    QG_HIP(hipGetLastError());
+}
+
+// ---- adjoint sweep ----------------------------------------------------------------------
+
+template <class R>
+struct SweepOp {
+    Mat2<R> udag;       // applied to phi and lambda
+    Mat2<R> gen;        // generator of a rotation (entries 0, +-1, +-i)
+    u64 gctrl;
+    unsigned lctrl;
+    short target;       // local bit (pair, diag-local) or state bit (diag-outer)
+    short kind;
+    int slot;           // gradient slot within the pass, -1 for fixed gates
+};
+
+struct SweepPass {
+    int m, low;
+    int op_begin, n_ops;
+    signed char pos[16];
+};
+
+// One pass of the adjoint backward sweep over a phi tile and a lambda tile: for each op in
+// order, the gradient term Im<lambda| C G |phi> of the tile (FP64, written to
+// partial[slot * n_tiles + tile]) and then phi, lambda <- U^dagger.
+template <class R, int MB>
+__global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__ phi, Cx<R> *__restrict__ lam,
+                                                             const SweepPass pass,
+                                                             const SweepOp<R> *__restrict__ ops,
+                                                             double *__restrict__ partial, u64 n_tiles,
+                                                             u64 first) {
+    __shared__ Cx<R> tp[1 << MB];
+    __shared__ Cx<R> tl[1 << MB];
+    __shared__ double scratch[kTileThreads / 64];
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const int m = pass.m;
+    const u64 tile = first + blockIdx.x;
+    const u64 base = tile_base(tile, pass.pos, m);
+    const int n_units = 1 << (m - lw);
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
+        const Amps<R> v = load_unit(phi, u), w = load_unit(lam, u);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            tp[l + j] = v.v[j];
+            tl[l + j] = w.v[j];
+        }
+    }
+    __syncthreads();
+    const int n_amps = 1 << m;
+    for (int k = 0; k < pass.n_ops; ++k) {
+        const SweepOp<R> &op = ops[pass.op_begin + k];
+        const bool on = (base & op.gctrl) == op.gctrl;
+        const unsigned lc = op.lctrl;
+        double s = 0.;
+        if (on && op.kind == kOpPair) {
+            const int tb = op.target;
+            const unsigned lowmask = (1u << tb) - 1;
+            for (int p = threadIdx.x; p < n_amps / 2; p += kTileThreads) {
+                const unsigned i0 = (((unsigned)p & ~lowmask) << 1) | ((unsigned)p & lowmask);
+                if ((i0 & lc) != lc) continue;
+                const unsigned i1 = i0 | (1u << tb);
+                const Cx<R> p0 = tp[i0], p1 = tp[i1], l0 = tl[i0], l1 = tl[i1];
+                if (op.slot >= 0) {
+                    const Cx<R> q0 = cadd(cmul(op.gen.m00, p0), cmul(op.gen.m01, p1));
+                    const Cx<R> q1 = cadd(cmul(op.gen.m10, p0), cmul(op.gen.m11, p1));
+                    s += (double)l0.re * (double)q0.im - (double)l0.im * (double)q0.re +
+                         (double)l1.re * (double)q1.im - (double)l1.im * (double)q1.re;
+                }
+                tp[i0] = cadd(cmul(op.udag.m00, p0), cmul(op.udag.m01, p1));
+                tp[i1] = cadd(cmul(op.udag.m10, p0), cmul(op.udag.m11, p1));
+                tl[i0] = cadd(cmul(op.udag.m00, l0), cmul(op.udag.m01, l1));
+                tl[i1] = cadd(cmul(op.udag.m10, l0), cmul(op.udag.m11, l1));
+            }
+        }
+        if (on && op.kind != kOpPair) {   // diagonal: per amplitude, factor by the target bit
+            const int tb = op.target;
+            for (int i = threadIdx.x; i < n_amps; i += kTileThreads) {
+                if (((unsigned)i & lc) != lc) continue;
+                const bool one = op.kind == kOpDiagLocal ? ((i >> tb) & 1) != 0 : ((base >> tb) & 1) != 0;
+                const Cx<R> x = tp[i], y = tl[i];
+                if (op.slot >= 0) {
+                    const Cx<R> q = cmul(one ? op.gen.m11 : op.gen.m00, x);
+                    s += (double)y.re * (double)q.im - (double)y.im * (double)q.re;
+                }
+                const Cx<R> f = one ? op.udag.m11 : op.udag.m00;
+                tp[i] = cmul(f, x);
+                tl[i] = cmul(f, y);
+            }
+        }
+        if (op.slot >= 0) {
+            s = block_sum_tile<kTileThreads>(s, scratch);
+            if (threadIdx.x == 0) partial[(u64)op.slot * n_tiles + tile] = s;
+        }
+        __syncthreads();
+    }
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
+        Amps<R> v, w;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            v.v[j] = tp[l + j];
+            w.v[j] = tl[l + j];
+        }
+        store_unit(phi, u, v);
+        store_unit(lam, u, w);
+    }
+}
+
+// terms[slot] (+)= sum over tiles of partial[slot * n_tiles + tile], fixed order; one
+// workgroup per slot.
+__global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, u64 n_tiles,
+                                                   double *__restrict__ terms) {
+    __shared__ double scratch[4];
+    const double *row = partial + (u64)blockIdx.x * n_tiles;
+    double s = 0.;
+    for (u64 i = threadIdx.x; i < n_tiles; i += 256) s += row[i];
+    s = block_sum_tile<256>(s, scratch);
+    if (threadIdx.x == 0) terms[blockIdx.x] = s;
+}
+
+// ---- Pauli sums ---------------------------------------------------------------------------
+
+template <class R>
+struct PauliCfg;
+template <>
+struct PauliCfg<float> {
+    static constexpr int MaxBits = 12;
+    static constexpr int Low = 5;
+};
+template <>
+struct PauliCfg<double> {
+    static constexpr int MaxBits = 11;
+    static constexpr int Low = 4;
+};
+
+struct PauliPass {
+    int m, low;
+    int g_begin, n_groups;
+    int accumulate;     // add to lambda (later passes) instead of writing it
+    int dot;            // last pass: partial[tile] = Re<psi|lambda> over the tile
+    signed char pos[16];
+};
+
+// lambda (+)= sum over this pass's groups g of sum_{k in g} c_k (-1)^popcount(x & zm_k) psi[x ^ xm_g]
+// for a tile holding every X bit of those groups; c_k carries the (-i)^(number of Y) factor.
+template <class R, int MB>
+__global__ __launch_bounds__(kTileThreads) void k_tile_pauli(const Cx<R> *__restrict__ psi, Cx<R> *__restrict__ lam,
+                                                             const PauliPass pass,
+                                                             const PauliGroup *__restrict__ groups,
+                                                             const PauliTerm *__restrict__ terms,
+                                                             double *__restrict__ partial, u64 first) {
+    __shared__ Cx<R> tp[1 << MB];
+    __shared__ double scratch[kTileThreads / 64];
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const int m = pass.m;
+    const u64 tile = first + blockIdx.x;
+    const u64 base = tile_base(tile, pass.pos, m);
+    const int n_units = 1 << (m - lw);
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        const Amps<R> v = load_unit(psi, (base | deposit(l, pass.pos, m, pass.low)) >> lw);
+#pragma unroll
+        for (int j = 0; j < W; ++j) tp[l + j] = v.v[j];
+    }
+    __syncthreads();
+    double dot = 0.;
+    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
+        const unsigned l = (unsigned)lu << lw;
+        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
+        Amps<R> out;
+        if (pass.accumulate) out = load_unit(lam, u);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const unsigned x = l + j;
+            const u64 gx = base | deposit(x, pass.pos, m, pass.low);
+            double ar = pass.accumulate ? (double)out.v[j].re : 0., ai = pass.accumulate ? (double)out.v[j].im : 0.;
+            for (int g = 0; g < pass.n_groups; ++g) {
+                const PauliGroup gr = groups[pass.g_begin + g];
+                double cr = 0., ci = 0.;
+                for (int k = gr.first; k < gr.first + gr.count; ++k) {
+                    const PauliTerm t = terms[k];
+                    const double sgn = (__popcll(gx & t.zmask) & 1) ? -1. : 1.;
+                    cr += sgn * t.cre;
+                    ci += sgn * t.cim;
+                }
+                const Cx<R> v = tp[x ^ gr.xlocal];
+                ar += cr * (double)v.re - ci * (double)v.im;
+                ai += cr * (double)v.im + ci * (double)v.re;
+            }
+            out.v[j] = cx<R>((R)ar, (R)ai);
+            if (pass.dot) dot += (double)tp[x].re * (double)out.v[j].re + (double)tp[x].im * (double)out.v[j].im;
+        }
+        store_unit(lam, u, out);
+    }
+    if (pass.dot) {
+        dot = block_sum_tile<kTileThreads>(dot, scratch);
+        if (threadIdx.x == 0) partial[tile] = dot;
+    }
+}
+
+template <class R>
+struct SweepCfg;
+template <>
+struct SweepCfg<float> {
+    static constexpr int MaxBits = 11;   // two 2^11 x 8 B tiles = 32 KiB of LDS
+    static constexpr int Low = 5;
+};
+template <>
+struct SweepCfg<double> {
+    static constexpr int MaxBits = 10;   // two 2^10 x 16 B tiles
+    static constexpr int Low = 4;
+};
+
+}  // namespace
+
+template <class R>
+void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int max_bits,
+                         int low_bits) {
+    typedef TileCfg<R> C;
+    const int mb = std::min(max_bits > 0 ? std::min(max_bits, C::MaxBits) : C::MaxBits, n_lanes);
+    const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
+    QG_REQUIRE(Unit<R>::W == 1 || low >= 1, "complex64 tiles need bit 0");
+    PassPlan plan;
+    auto open = [&]() {
+        plan.bits.clear();
+        plan.gates.clear();
+        for (int b = 0; b < low; ++b) plan.bits.push_back(b);
+    };
+    open();
+    for (const QueuedGate &g : gates) {
+        const bool need = !(is_diagonal(g.m) || plan.has(g.target));
+        if ((int)plan.bits.size() + (need ? 1 : 0) > mb || (int)plan.gates.size() >= C::MaxOps) {
+            launch_pass<R>(a, n_lanes, plan, low, mb);
+            open();
+        }
+        if (!is_diagonal(g.m) && !plan.has(g.target)) plan.bits.push_back(g.target);
+        plan.gates.push_back(&g);
+    }
+    if (!plan.gates.empty()) launch_pass<R>(a, n_lanes, plan, low, mb);
+}
+
+template void apply_gate_sequence<float>(Cx<float> *, int, const std::vector<QueuedGate> &, int, int);
+template void apply_gate_sequence<double>(Cx<double> *, int, const std::vector<QueuedGate> &, int, int);
+
+template <class R>
+void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepGate> &gates, double *d_terms,
+                   int max_bits, int low_bits) {
+    typedef SweepCfg<R> C;
+    const int mb = std::min(max_bits > 0 ? std::min(max_bits, C::MaxBits) : C::MaxBits, n_lanes);
+    const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
+    // plan the passes: like apply_gate_sequence, without a limit on gates per pass
+    struct Plan {
+        std::vector<int> bits;
+        int begin, end;
+    };
+    std::vector<Plan> plans;
+    Plan cur;
+    auto open = [&](int at) {
+        cur.bits.clear();
+        for (int b = 0; b < low; ++b) cur.bits.push_back(b);
+        cur.begin = cur.end = at;
+    };
+    auto has = [&](int b) { return std::find(cur.bits.begin(), cur.bits.end(), b) != cur.bits.end(); };
+    open(0);
+    for (int i = 0; i < (int)gates.size(); ++i) {
+        const SweepGate &g = gates[i];
+        const bool diag = is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3);
+        const bool need = !(diag || has(g.target));
+        if ((int)cur.bits.size() + (need ? 1 : 0) > mb) {
+            plans.push_back(cur);
+            open(i);
+        }
+        if (need) cur.bits.push_back(g.target);
+        cur.end = i + 1;
+    }
+    if (cur.end > cur.begin) plans.push_back(cur);
+
+    // device op array, pass by pass; slots numbered within each pass
+    std::vector<SweepOp<R>> hops(gates.size());
+    std::vector<SweepPass> passes;
+    std::vector<std::vector<int>> slot_gate;   // per pass: gate index of each slot
+    int max_slots = 0;
+    for (const Plan &pl : plans) {
+        std::vector<int> bits(pl.bits);
+        for (int b = 0; (int)bits.size() < mb && b < n_lanes; ++b)
+            if (std::find(bits.begin(), bits.end(), b) == bits.end()) bits.push_back(b);
+        std::sort(bits.begin(), bits.end());
+        SweepPass ps;
+        ps.m = (int)bits.size();
+        ps.low = low;
+        ps.op_begin = pl.begin;
+        ps.n_ops = pl.end - pl.begin;
+        for (int j = 0; j < 16; ++j) ps.pos[j] = j < ps.m ? (signed char)bits[j] : (signed char)0;
+        int local[64];
+        for (int b = 0; b < 64; ++b) local[b] = -1;
+        for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
+        std::vector<int> slots;
+        for (int i = pl.begin; i < pl.end; ++i) {
+            const SweepGate &g = gates[i];
+            SweepOp<R> &op = hops[i];
+            op.udag = round_matrix<R>(g.udag);
+            GateMatrix gen;
+            set_pauli(gen, g.pauli);
+            op.gen = round_matrix<R>(gen);
+            op.gctrl = 0;
+            op.lctrl = 0;
+            for (int c : g.ctrl) {
+                if (local[c] >= 0)
+                    op.lctrl |= 1u << local[c];
+                else
+                    op.gctrl |= u64(1) << c;
+            }
+            const bool diag = is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3);
+            if (local[g.target] >= 0) {
+                op.target = (short)local[g.target];
+                op.kind = (short)(diag ? kOpDiagLocal : kOpPair);
+            } else {
+                QG_REQUIRE(diag, "adjoint sweep: non-diagonal gate outside the tile");
+                op.target = (short)g.target;
+                op.kind = (short)kOpDiagOuter;
+            }
+            op.slot = g.pauli != 0 ? (int)slots.size() : -1;
+            if (g.pauli != 0) slots.push_back(i);
+        }
+        max_slots = std::max(max_slots, (int)slots.size());
+        passes.push_back(ps);
+        slot_gate.push_back(slots);
+    }
+    if (passes.empty()) return;
+
+    hipStream_t s = stream();
+    const size_t ops_bytes = sizeof(SweepOp<R>) * hops.size();
+    SweepOp<R> *d_ops = static_cast<SweepOp<R> *>(dev_alloc(ops_bytes));
+    size_t part_bytes = 0;
+    for (const SweepPass &ps : passes)
+        part_bytes = std::max(part_bytes, sizeof(double) * (size_t)max_slots * (size_t)(u64(1) << (n_lanes - ps.m)));
+    double *d_part = static_cast<double *>(dev_alloc(std::max<size_t>(part_bytes, 8)));
+    hipError_t e = hipMemcpyAsync(d_ops, hops.data(), ops_bytes, hipMemcpyHostToDevice, s);
+    for (size_t pi = 0; pi < passes.size() && e == hipSuccess; ++pi) {
+        const SweepPass &ps = passes[pi];
+        const u64 n_tiles = u64(1) << (n_lanes - ps.m);
+        const u64 piece = launch_piece_items() / kTileThreads;
+        for (u64 first = 0; first < n_tiles; first += piece) {
+            const u64 count = std::min(piece, n_tiles - first);
+            hipLaunchKernelGGL((k_tile_sweep<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
+                               d_ops, d_part, n_tiles, first);
+        }
+        const std::vector<int> &slots = slot_gate[pi];
+        if (!slots.empty()) {
+            // terms of this pass's slots land at d_terms[gate index]
+            for (size_t k = 0; k < slots.size(); ++k)
+                hipLaunchKernelGGL(k_sum_slots, 1, 256, 0, s, d_part + k * n_tiles, n_tiles, d_terms + slots[k]);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // hops is a host temporary
+    dev_free(d_part, std::max<size_t>(part_bytes, 8));
+    dev_free(d_ops, ops_bytes);
+    QG_HIP(e);
+}
+
+template void adjoint_sweep<float>(Cx<float> *, Cx<float> *, int, const std::vector<SweepGate> &, double *, int,
+                                   int);
+template void adjoint_sweep<double>(Cx<double> *, Cx<double> *, int, const std::vector<SweepGate> &, double *,
+                                    int, int);
+
+
+template <class R>
+double pauli_expectation(const Cx<R> *psi, Cx<R> *lam, int n_lanes, const std::vector<PauliGroup> &groups_in,
+                         const std::vector<PauliTerm> &terms) {
+    typedef PauliCfg<R> C;
+    const int mb = std::min(C::MaxBits, n_lanes);
+    const int low = std::min(C::Low, mb);
+    // passes: consecutive groups whose X bits fit one tile together
+    std::vector<PauliGroup> groups(groups_in);
+    struct Plan {
+        std::vector<int> bits;
+        int begin, end;
+    };
+    std::vector<Plan> plans;
+    Plan cur;
+    auto open = [&](int at) {
+        cur.bits.clear();
+        for (int b = 0; b < low; ++b) cur.bits.push_back(b);
+        cur.begin = cur.end = at;
+    };
+    auto missing = [&](u64 xm) {
+        std::vector<int> need;
+        for (int b = 0; b < n_lanes; ++b)
+            if (((xm >> b) & 1) && std::find(cur.bits.begin(), cur.bits.end(), b) == cur.bits.end())
+                need.push_back(b);
+        return need;
+    };
+    const u64 always = Unit<R>::W == 2 ? 1 : 0;   // complex64 tiles always hold bit 0
+    open(0);
+    for (int i = 0; i < (int)groups.size(); ++i) {
+        QG_REQUIRE(__builtin_popcountll(groups[i].xmask | always) <= mb,
+                   "Pauli term with %d X/Y factors does not fit a %d-bit tile", __builtin_popcountll(groups[i].xmask), mb);
+        std::vector<int> need = missing(groups[i].xmask);
+        if ((int)(cur.bits.size() + need.size()) > mb) {
+            if (cur.end > cur.begin) plans.push_back(cur);
+            open(i);
+            need = missing(groups[i].xmask);
+            if ((int)(cur.bits.size() + need.size()) > mb) {   // a wide term: give up the low rows
+                cur.bits.clear();
+                need = missing(groups[i].xmask);
+            }
+        }
+        for (int b : need) cur.bits.push_back(b);
+        cur.end = i + 1;
+    }
+    plans.push_back(cur);
+
+    std::vector<PauliPass> passes;
+    for (const Plan &pl : plans) {
+        std::vector<int> bits(pl.bits);
+        // complex64 tiles move 2-amplitude units: bit 0 must be a tile bit
+        if (Unit<R>::W == 2 && std::find(bits.begin(), bits.end(), 0) == bits.end()) bits.push_back(0);
+        for (int b = 0; (int)bits.size() < mb && b < n_lanes; ++b)
+            if (std::find(bits.begin(), bits.end(), b) == bits.end()) bits.push_back(b);
+        QG_REQUIRE((int)bits.size() <= mb, "Pauli term does not fit a tile");
+        std::sort(bits.begin(), bits.end());
+        PauliPass ps;
+        ps.m = (int)bits.size();
+        int lowrun = 0;
+        while (lowrun < ps.m && bits[lowrun] == lowrun) ++lowrun;
+        ps.low = lowrun;
+        ps.g_begin = pl.begin;
+        ps.n_groups = pl.end - pl.begin;
+        ps.accumulate = passes.empty() ? 0 : 1;
+        ps.dot = 0;
+        for (int j = 0; j < 16; ++j) ps.pos[j] = j < ps.m ? (signed char)bits[j] : (signed char)0;
+        int local[64];
+        for (int b = 0; b < 64; ++b) local[b] = -1;
+        for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
+        for (int i = pl.begin; i < pl.end; ++i) {
+            unsigned xl = 0;
+            for (int b = 0; b < n_lanes; ++b)
+                if ((groups[i].xmask >> b) & 1) {
+                    QG_REQUIRE(local[b] >= 0, "Pauli group outside its tile");
+                    xl |= 1u << local[b];
+                }
+            groups[i].xlocal = xl;
+        }
+        passes.push_back(ps);
+    }
+    passes.back().dot = 1;
+
+    hipStream_t s = stream();
+    const size_t gbytes = sizeof(PauliGroup) * std::max<size_t>(1, groups.size());
+    const size_t tbytes = sizeof(PauliTerm) * std::max<size_t>(1, terms.size());
+    const u64 n_tiles_last = u64(1) << (n_lanes - passes.back().m);
+    const size_t pbytes = sizeof(double) * (size_t)(n_tiles_last + 1);
+    char *buf = static_cast<char *>(dev_alloc(gbytes + tbytes + pbytes));
+    PauliGroup *d_groups = reinterpret_cast<PauliGroup *>(buf);
+    PauliTerm *d_terms = reinterpret_cast<PauliTerm *>(buf + gbytes);
+    double *d_part = reinterpret_cast<double *>(buf + gbytes + tbytes);
+    Runtime &rt = runtime();
+    hipError_t e = hipMemcpyAsync(d_groups, groups.data(), sizeof(PauliGroup) * groups.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_terms, terms.data(), sizeof(PauliTerm) * terms.size(), hipMemcpyHostToDevice, s);
+    for (size_t pi = 0; pi < passes.size() && e == hipSuccess; ++pi) {
+        const PauliPass &ps = passes[pi];
+        const u64 n_tiles = u64(1) << (n_lanes - ps.m);
+        const u64 piece = launch_piece_items() / kTileThreads;
+        for (u64 first = 0; first < n_tiles; first += piece) {
+            const u64 count = std::min(piece, n_tiles - first);
+            hipLaunchKernelGGL((k_tile_pauli<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, psi, lam, ps,
+                               d_groups, d_terms, d_part, first);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sum_slots, 1, 256, 0, s, d_part, n_tiles_last, d_part + n_tiles_last);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rt.h_pinned, d_part + n_tiles_last, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // group and term tables are host temporaries
+    dev_free(buf, gbytes + tbytes + pbytes);
+    QG_HIP(e);
+    return rt.h_pinned[0];
+}
+
+template double pauli_expectation<float>(const Cx<float> *, Cx<float> *, int, const std::vector<PauliGroup> &,
+                                         const std::vector<PauliTerm> &);
+template double pauli_expectation<double>(const Cx<double> *, Cx<double> *, int, const std::vector<PauliGroup> &,
+                                          const std::vector<PauliTerm> &);
+
+}  // namespace qg

@@ -22,6 +22,54 @@ template <class R>
 void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
                     const std::vector<std::complex<double>> &mat);
 
+// A gate waiting in a fusion queue: FP64 matrix, control lanes, target lane.
+struct QueuedGate {
+    GateMatrix m;
+    std::vector<int> ctrl;
+    int target;
+};
+
+// Apply `gates` in order in tile passes (k_tile.hip): each pass loads tiles of up to 2^max_bits
+// amplitudes (<= 0: the precision's default) whose lowest `low_bits` bits are contiguous
+// (< 0: default) and applies every gate it can hold before writing them back.
+template <class R>
+void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int max_bits,
+                         int low_bits);
+
+// One step of the adjoint backward sweep: U^dagger (FP64) with its controls and target, and
+// the generator of a rotation (pauli 1 X, 2 Y, 3 Z; 0 for a fixed gate).
+struct SweepGate {
+    GateMatrix udag;
+    int pauli;
+    std::vector<int> ctrl;
+    int target;
+};
+
+// Backward sweep in tile passes over phi and lambda: for gate i in order, d_terms[i] =
+// Im<lambda| C G |phi> (rotations only; FP64, summed over tiles in a fixed order), then
+// phi, lambda <- U^dagger. Synchronises.
+template <class R>
+void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepGate> &gates, double *d_terms,
+                   int max_bits, int low_bits);
+
+// Pauli-sum observable: terms with the same X mask form a group; cre/cim carry the term's
+// coefficient times (-i)^(number of Y factors); xlocal is filled in by the launcher.
+struct PauliTerm {
+    double cre, cim;
+    u64 zmask;
+};
+struct PauliGroup {
+    u64 xmask;
+    int first, count;
+    unsigned xlocal;
+};
+
+// lambda = H psi in tile passes (groups whose X bits fit one tile share a pass) and
+// <psi|H|psi> = Re<psi|lambda> (FP64, fixed order) from the last pass. Synchronises.
+template <class R>
+double pauli_expectation(const Cx<R> *psi, Cx<R> *lam, int n_lanes, const std::vector<PauliGroup> &groups,
+                         const std::vector<PauliTerm> &terms);
+
 // ---- state -----------------------------------------------------------------------------
 template <class R> void launch_set_zero_state(Cx<R> *a, int n_lanes);
 // P(bit `lane` == 0), FP64 accumulation; synchronises.

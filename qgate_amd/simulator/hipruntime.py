@@ -65,6 +65,7 @@ def lib():
         'qghip_qubit_processor_new': (ctypes.c_int, [ctypes.c_int, _c_void_pp]),
         'qghip_qubit_processor_delete': (ctypes.c_int, [H]),
         'qghip_qubit_processor_synchronize': (ctypes.c_int, [H]),
+        'qghip_qubit_processor_set_fusion': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         'qghip_qubit_processor_reset': (ctypes.c_int, [H]),
         'qghip_qubit_processor_initialize_qubit_states': (ctypes.c_int, [H, H, ctypes.c_int]),
         'qghip_qubit_processor_reset_qubit_states': (ctypes.c_int, [H, H]),
@@ -130,6 +131,8 @@ def _handles(objs):
 this.initialized = False
 this.device_id = 0
 this.native_instances = weakref.WeakValueDictionary()
+_FUSION_MODES = {'off': 0, 'tile': 1}
+this.fusion = ('off', 0, -1)
 
 
 def set_preference(device_id=None):
@@ -137,6 +140,20 @@ def set_preference(device_id=None):
         raise RuntimeError('already initialized.')
     if device_id is not None:
         this.device_id = int(device_id)
+
+
+def set_fusion(mode='tile', tile_bits=0, low_bits=-1):
+    """Gate fusion of the processors created from now on and of the live ones: 'tile' queues
+    the gates of each state vector and applies them in tile passes (many gates per trip
+    through HBM; tiles of 2^tile_bits amplitudes whose lowest low_bits bits are contiguous,
+    0 / -1: defaults); 'off' applies every gate when it is issued."""
+    if mode not in _FUSION_MODES:
+        raise ValueError('fusion mode must be one of {}'.format(sorted(_FUSION_MODES)))
+    this.fusion = (mode, int(tile_bits), int(low_bits))
+    for obj in list(this.native_instances.values()):
+        proc = getattr(obj, 'processor', None)
+        if proc is not None and getattr(proc, 'ptr', None) is not None:
+            proc.set_fusion(*this.fusion)
 
 
 def reset_preference():
@@ -209,6 +226,10 @@ class HipQubitProcessor:
         h = ctypes.c_void_p()
         check(lib().qghip_qubit_processor_new(_prec(dtype), ctypes.byref(h)))
         self.ptr = h.value
+        self.set_fusion(*this.fusion)
+
+    def set_fusion(self, mode, tile_bits=0, low_bits=-1):
+        check(lib().qghip_qubit_processor_set_fusion(self.ptr, _FUSION_MODES[mode], tile_bits, low_bits))
 
     def __del__(self):
         self.delete()

@@ -115,10 +115,10 @@ def test_hea_gradient_vs_parameter_shift(hip_module, port_module, dtype):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('seed', [0, 1, 2])
-def test_mixed_gradient_vs_central_differences(hip_module, port_module, seed):
-    """Controlled rotations, fixed gates (adjoints too), shared parameters, X/Y/Z terms."""
-    n = 7
+@pytest.mark.parametrize('seed,n', [(0, 7), (1, 7), (2, 7), (3, 13), (4, 14)])
+def test_mixed_gradient_vs_central_differences(hip_module, port_module, seed, n):
+    """Controlled rotations, fixed gates (adjoints too), shared parameters, X/Y/Z terms; at 13
+    and 14 qubits the tiles (2^11 / 2^10 amplitudes) leave some targets and controls outside."""
     spec = ac.mixed_spec(n, 60, seed=seed, n_params=15)
     h = ac.random_observable(n, 10, seed=seed + 50)
     params = np.random.RandomState(seed + 70).uniform(-math.pi, math.pi, spec.n_params)
@@ -130,6 +130,26 @@ def test_mixed_gradient_vs_central_differences(hip_module, port_module, seed):
     err = np.max(np.abs(grad - ref))
     print('seed', seed, 'grad err', err)
     assert err < 1e-7, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('n', [5, 12, 15])
+def test_tile_passes_match_per_gate_launches(hip_module, dtype, n):
+    """The tile-pass forward and backward sweep against one launch per gate and per step."""
+    spec = ac.mixed_spec(n, 120, seed=n, n_params=30)
+    h = ac.random_observable(n, 12, seed=n + 1)
+    params = np.random.RandomState(n + 2).uniform(-math.pi, math.pi, spec.n_params)
+    out = {}
+    for mode in ('off', 'tile'):
+        circ = spec.build(dtype).set_fusion(mode)
+        value, grad = circ.gradient(params, h)
+        circ.expectation(params, h)
+        out[mode] = (value, grad, circ.state())
+    tol = 1e-12 if dtype == np.float64 else 2e-4
+    assert abs(out['off'][0] - out['tile'][0]) < tol
+    assert np.max(np.abs(out['off'][1] - out['tile'][1])) < tol
+    assert np.max(np.abs(out['off'][2] - out['tile'][2])) < (1e-12 if dtype == np.float64 else 5e-6)
 
 
 @pytest.mark.gpu
@@ -208,3 +228,31 @@ def test_argument_errors(hip_module):
     with pytest.raises(ValueError):
         circ.expectation([0., 0., 0.], PauliSum([(1., {5: 'Z'})]))  # term outside register
     assert abs(circ.expectation([0., 0., math.pi / 3], PauliSum([(1., {0: 'Z'})])) - 0.5) < 1e-15
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('n', [1, 2, 3])
+def test_small_registers(hip_module, port_module, dtype, n):
+    """Registers smaller than a tile row: X, Y and Z terms on every qubit."""
+    spec = ac.CircuitSpec(n)
+    k = 0
+    for layer in range(3):
+        for q in range(n):
+            spec.rx(q, k)
+            spec.ry(q, k + 1)
+            spec.rz(q, k + 2)
+            k += 3
+        for q in range(n - 1):
+            spec.cz(q, q + 1)
+    terms = []
+    for q in range(n):
+        terms += [(0.7, {q: 'X'}), (-0.4, {q: 'Y'}), (0.3, {q: 'Z'})]
+    if n >= 2:
+        terms.append((0.9, {0: 'Y', n - 1: 'X'}))
+    h = PauliSum(terms)
+    params = np.random.RandomState(n).uniform(-math.pi, math.pi, spec.n_params)
+    value, grad = spec.build(dtype).gradient(params, h)
+    ref = ac.parameter_shift_gradient(port_module, spec, params, h)
+    assert abs(value - ac.oracle_expectation(port_module, spec, params, h)) < VALUE_TOL[dtype] * 10
+    assert np.max(np.abs(grad - ref)) < GRAD_TOL[dtype]

