@@ -13,7 +13,8 @@ uncontrolled 2x2 gate 2 S 2^n, c-controlled 2 S 2^(n-c), diagonal gates with m00
 
 roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, timed with
 HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
-launch time, against 8 TB/s.
+launch time, against 8 TB/s; traffic = HBM bytes per launch measured with the FETCH_SIZE and
+WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md).
 
 batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), each with its
 own parameters and observable, <H> and gradients in one graph replay, against a loop of
@@ -112,6 +113,20 @@ def cpu_baseline(gates, n, amp, budget_s=15.0):
                       '{:.3f} s/gate'.format(done, dt, dt / max(done, 1))}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the gate kernels at 30 qubits complex64, from the FETCH_SIZE /
+    WRITE_SIZE counter runs summarised in profiles/traffic_gate_30q_c64.md (scaled by a copy
+    kernel of known size); None if that summary is absent."""
+    path = os.path.join(ROOT, 'profiles', 'traffic_gate_30q_c64.md')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith('traffic_bytes_per_launch:'):
+                return int(line.split(':')[1])
+    return None
+
+
 def roofline(runner, n, amp, reps=4):
     """Mean launch time of the uncontrolled 2x2 pair kernel over every target bit."""
     from qgate_amd.simulator import hipruntime
@@ -128,7 +143,8 @@ def roofline(runner, n, amp, reps=4):
         ms = float(np.mean([per_target[t] for t in rng]))
         split[name] = round(launch_bytes / (ms * 1e-3) / 1e9, 1)
     return {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+            'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'traffic': measured_traffic() if (n, amp) == (30, 8) else None,
             'kernel': 'uncontrolled 2x2 gate (k_unit_pair / k_wave_pair / k_strided_pair), complex64, 30 qubits',
             'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
             'by_kernel_GBs': split,

@@ -253,6 +253,10 @@ int qghip_batch_describe(const qghip_batch *b, char *buf, int size);
  * below the 2^32 work-item limit at 33+ qubits. Results do not depend on it; tests lower it
  * to run the piecewise path on small states. */
 int qghip_set_launch_piece(int64_t items);
+/* Full-size tile passes keep the tile in registers, going through LDS once per phase of
+ * gates (on, the default), or apply every gate to the tile in LDS (off). Results agree to
+ * rounding; tests compare the two. */
+int qghip_set_register_tiles(int on);
 
 /* ---- measurement helper (no reference counterpart) ------------------------------------ */
 

@@ -218,6 +218,10 @@ int qghip_set_launch_piece(int64_t items) {
     });
 }
 
+int qghip_set_register_tiles(int on) {
+    return guarded([&] { set_register_tiles(on != 0); });
+}
+
 int qghip_device_info(int *device_id, int *n_cus, uint64_t *total_mem) {
     return guarded([&] {
         stream();

@@ -639,10 +639,16 @@ void build_group(Group &g, const Tuning &t) {
     }
 
     // device memory
-    free_group(g);
-    g.state_bytes = (amp_bytes(Unit<R>::W == 2 ? precFP32 : precFP64) << n) * ni;
-    g.psi = dev_alloc(g.state_bytes);
-    g.lam = dev_alloc(g.state_bytes);
+    const size_t state_bytes = (amp_bytes(Unit<R>::W == 2 ? precFP32 : precFP64) << n) * ni;
+    if (g.psi == nullptr || g.state_bytes != state_bytes) {
+        free_group(g);
+        g.state_bytes = state_bytes;
+        g.psi = dev_alloc(g.state_bytes);
+        g.lam = dev_alloc(g.state_bytes);
+    } else if (g.block) {
+        dev_free(g.block, g.block_bytes);
+        g.block = nullptr;
+    }
     u64 max_part = 0;
     for (const BPass &p : g.spasses)
         max_part = std::max<u64>(max_part, (u64)ni * (u64)max_slots * (u64(1) << (n - p.m)));
@@ -812,19 +818,40 @@ void tune_group(qghip_batch *b, Group &g) {
     }
     std::vector<int> threads = b->fixed_threads > 0 ? std::vector<int>{b->fixed_threads} : std::vector<int>{128, 256};
     std::vector<int> grids = b->fixed_grid > 0 ? std::vector<int>{b->fixed_grid} : std::vector<int>{2, 4, 8};
+    // time the candidates on a sample of the group's items (the structure is the same for
+    // every item, so the ranking carries over); the sample still spans several workgroups
+    // per compute unit
+    Group sample;
+    sample.n = g.n;
+    sample.key = g.key;
+    sample.ops = g.ops;
+    sample.n_params = g.n_params;
+    sample.xmask = g.xmask;
+    sample.zmask = g.zmask;
+    sample.pgroups = g.pgroups;
+    const size_t keep = std::min<size_t>(g.n_items(), 64);
+    sample.items.assign(g.items.begin(), g.items.begin() + keep);
+    sample.params.assign(g.params.begin(), g.params.begin() + keep * (size_t)g.n_params);
+    sample.coefs.assign(g.coefs.begin(), g.coefs.begin() + keep * g.zmask.size() * 2);
     Tuning best;
     best.ms = -1.;
-    for (int tb : bits)
-        for (int th : threads)
-            for (int gr : grids) {
-                Tuning t;
-                t.tile_bits = tb;
-                t.threads = th;
-                t.grid_per_cu = gr;
-                build_group<R>(g, t);
-                t.ms = time_group<R>(g, 2);
-                if (best.ms < 0. || t.ms < best.ms) best = t;
-            }
+    try {
+        for (int tb : bits)
+            for (int th : threads)
+                for (int gr : grids) {
+                    Tuning t;
+                    t.tile_bits = tb;
+                    t.threads = th;
+                    t.grid_per_cu = gr;
+                    build_group<R>(sample, t);
+                    t.ms = time_group<R>(sample, 2);
+                    if (best.ms < 0. || t.ms < best.ms) best = t;
+                }
+    } catch (...) {
+        free_group(sample);
+        throw;
+    }
+    free_group(sample);
     build_group<R>(g, best);
     g.tune = best;
 }

@@ -13,6 +13,7 @@
 // (CPUQubitProcessor.cpp:307-362): a[i0] <- m00 a[i0] + m01 a[i1], a[i1] <- m10 a[i0] + m11 a[i1].
 // The gate list of a pass travels in the kernel arguments (at most 40 / 24 gates).
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "kernels.hpp"
@@ -118,6 +119,203 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
 
 bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
 
+// ---- register-resident tile passes ------------------------------------------------------
+//
+// The full-size pass (m == MaxBits) keeps the tile in registers: each of the T threads holds
+// A = 2^m / T amplitudes. A pass is split into phases; in each phase RB = log2(A) tile bits
+// are register bits (the targets of the phase's non-diagonal gates), 6 are lane bits and the
+// rest wave bits, so the phase's pair updates are register-to-register with no LDS traffic
+// or barriers. Between phases the tile goes through LDS once (write, barrier, read in the
+// next phase's layout). A pass of g gates costs about g / RB LDS round trips instead of g.
+
+constexpr int kMaxPhases = 40;
+
+struct RegPhase {
+    signed char rb[5];   // local bit of register bit k (k < RB)
+    signed char lb[6];   // local bit of lane bit k
+    signed char wb[3];   // local bit of wave bit k
+    unsigned char op_begin, n_ops;
+};
+
+template <class R>
+struct RegProgram {
+    int n_ops, n_phases, low;
+    signed char pos[16];
+    RegPhase phases[kMaxPhases];
+    TileOp<R> ops[TileCfg<R>::MaxOps];   // pair gates: target = register position
+};
+
+template <int A>
+struct Log2 {
+    static constexpr int v = A <= 1 ? 0 : 1 + Log2<A / 2>::v;
+};
+template <>
+struct Log2<1> {
+    static constexpr int v = 0;
+};
+
+template <class R, int A, int P>
+__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc) {
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        if ((j >> P) & 1) continue;
+        constexpr int step = 1 << P;
+        if ((lidx[j] & lc) == lc) {
+            const Cx<R> x0 = v[j], x1 = v[j + step];
+            v[j] = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
+            v[j + step] = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
+        }
+    }
+}
+
+template <class R, int A>
+__device__ __forceinline__ void reg_pair_at(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc,
+                                            int p) {
+    constexpr int RB = Log2<A>::v;
+    if constexpr (RB > 0) {
+        if (p == 0) reg_pair<R, A, 0>(v, lidx, mm, lc);
+    }
+    if constexpr (RB > 1) {
+        if (p == 1) reg_pair<R, A, 1>(v, lidx, mm, lc);
+    }
+    if constexpr (RB > 2) {
+        if (p == 2) reg_pair<R, A, 2>(v, lidx, mm, lc);
+    }
+    if constexpr (RB > 3) {
+        if (p == 3) reg_pair<R, A, 3>(v, lidx, mm, lc);
+    }
+    if constexpr (RB > 4) {
+        if (p == 4) reg_pair<R, A, 4>(v, lidx, mm, lc);
+    }
+}
+
+template <class R, int M, int T>
+__global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const RegProgram<R> prog, u64 first) {
+    constexpr int A = (1 << M) / T, RB = Log2<A>::v, WB = Log2<T / 64>::v;
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    __shared__ Cx<R> t[1 << M];
+    const u64 base = tile_base(first + blockIdx.x, prog.pos, M);
+    constexpr int n_units = 1 << (M - lw);
+    for (int lu = threadIdx.x; lu < n_units; lu += T) {
+        const unsigned l = (unsigned)lu << lw;
+        const Amps<R> x = load_unit(a, (base | deposit(l, prog.pos, M, prog.low)) >> lw);
+#pragma unroll
+        for (int j = 0; j < W; ++j) t[l + j] = x.v[j];
+    }
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Cx<R> v[A];
+    unsigned lidx[A];
+    for (int ph = 0; ph < prog.n_phases; ++ph) {
+        const RegPhase P = prog.phases[ph];
+        unsigned off = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb[k];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb[k];
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            unsigned l = off;
+#pragma unroll
+            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb[k];
+            lidx[j] = l;
+            v[j] = t[l];
+        }
+        for (int k = 0; k < P.n_ops; ++k) {
+            const TileOp<R> &op = prog.ops[P.op_begin + k];
+            if ((base & op.gctrl) != op.gctrl) continue;   // same for the whole workgroup
+            const unsigned lc = op.lctrl;
+            if (op.kind == kOpPair) reg_pair_at<R, A>(v, lidx, op.m, lc, op.target);
+            if (op.kind == kOpDiagLocal) {
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if ((lidx[j] & lc) == lc) v[j] = cmul(((lidx[j] >> op.target) & 1u) ? op.m.m11 : op.m.m00, v[j]);
+            }
+            if (op.kind == kOpDiagOuter) {
+                const Cx<R> f = ((base >> op.target) & 1) ? op.m.m11 : op.m.m00;
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if ((lidx[j] & lc) == lc) v[j] = cmul(f, v[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j) t[lidx[j]] = v[j];
+        __syncthreads();
+    }
+    for (int lu = threadIdx.x; lu < n_units; lu += T) {
+        const unsigned l = (unsigned)lu << lw;
+        Amps<R> x;
+#pragma unroll
+        for (int j = 0; j < W; ++j) x.v[j] = t[l + j];
+        store_unit(a, (base | deposit(l, prog.pos, M, prog.low)) >> lw, x);
+    }
+}
+
+// Phases of a pass: consecutive gates whose pair targets (pair[k] >= 0: local bit of a
+// non-diagonal gate) fit rb_count register bits. Each phase also fixes its lane bits (the
+// lowest 6 other tile bits) and wave bits (the rest).
+struct PhaseLayout {
+    std::vector<int> regs, lanes, waves;
+    int begin, end;
+};
+
+std::vector<PhaseLayout> plan_phase_layouts(const std::vector<int> &pair, int m, int rb_count, int wb_count) {
+    std::vector<PhaseLayout> out;
+    int k = 0;
+    const int n = (int)pair.size();
+    while (k < n) {
+        PhaseLayout P;
+        P.begin = k;
+        for (; k < n; ++k) {
+            if (pair[k] < 0) continue;
+            if (std::find(P.regs.begin(), P.regs.end(), pair[k]) != P.regs.end()) continue;
+            if ((int)P.regs.size() == rb_count) break;
+            P.regs.push_back(pair[k]);
+        }
+        P.end = k;
+        for (int b = 0; (int)P.regs.size() < rb_count && b < m; ++b)
+            if (std::find(P.regs.begin(), P.regs.end(), b) == P.regs.end()) P.regs.push_back(b);
+        std::sort(P.regs.begin(), P.regs.end());
+        for (int b = 0; b < m; ++b)
+            if (std::find(P.regs.begin(), P.regs.end(), b) == P.regs.end()) {
+                if ((int)P.lanes.size() < 6)
+                    P.lanes.push_back(b);
+                else
+                    P.waves.push_back(b);
+            }
+        QG_REQUIRE((int)P.waves.size() == wb_count, "tile layout does not match the workgroup");
+        out.push_back(P);
+    }
+    return out;
+}
+
+int reg_position(const PhaseLayout &P, int bit) {
+    return (int)(std::find(P.regs.begin(), P.regs.end(), bit) - P.regs.begin());
+}
+
+// Phase layouts of a register program: ops[k].target holds local bits on entry; pair gates
+// get their register position. Returns false if the pass needs more phases than it holds.
+template <class R>
+bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
+    std::vector<int> pair(prog.n_ops);
+    for (int k = 0; k < prog.n_ops; ++k) pair[k] = prog.ops[k].kind == kOpPair ? prog.ops[k].target : -1;
+    const std::vector<PhaseLayout> L = plan_phase_layouts(pair, m, rb_count, wb_count);
+    if ((int)L.size() > kMaxPhases) return false;
+    prog.n_phases = (int)L.size();
+    for (int i = 0; i < prog.n_phases; ++i) {
+        RegPhase &P = prog.phases[i];
+        std::memset(&P, 0, sizeof(P));
+        for (int b = 0; b < rb_count; ++b) P.rb[b] = (signed char)L[i].regs[b];
+        for (int b = 0; b < 6; ++b) P.lb[b] = (signed char)L[i].lanes[b];
+        for (int b = 0; b < wb_count; ++b) P.wb[b] = (signed char)L[i].waves[b];
+        P.op_begin = (unsigned char)L[i].begin;
+        P.n_ops = (unsigned char)(L[i].end - L[i].begin);
+        for (int k = L[i].begin; k < L[i].end; ++k)
+            if (prog.ops[k].kind == kOpPair) prog.ops[k].target = (short)reg_position(L[i], prog.ops[k].target);
+    }
+    return true;
+}
+
 // Pauli X (1), Y (2), Z (3); zero matrix otherwise.
 void set_pauli(GateMatrix &g, int pauli) {
     const std::complex<double> J(0., 1.);
@@ -189,6 +387,23 @@ void launch_pass(Cx<R> *a, int n_lanes, const PassPlan &plan, int low, int max_b
     hipStream_t s = stream();
     // one workgroup per tile; pieces keep every grid within launch_piece_items() threads
     const u64 piece = launch_piece_items() / kTileThreads;
+    if (prog.m == C::MaxBits && use_register_tiles()) {
+        RegProgram<R> rp;
+        rp.n_ops = prog.n_ops;
+        rp.low = prog.low;
+        std::memcpy(rp.pos, prog.pos, sizeof(rp.pos));
+        for (int k = 0; k < prog.n_ops; ++k) rp.ops[k] = prog.ops[k];
+        constexpr int A = (1 << C::MaxBits) / kTileThreads;
+        if (plan_phases<R>(rp, prog.m, Log2<A>::v, Log2<kTileThreads / 64>::v)) {
+            for (u64 first = 0; first < n_tiles; first += piece) {
+                const u64 count = std::min(piece, n_tiles - first);
+                hipLaunchKernelGGL((k_tile_reg<R, C::MaxBits, kTileThreads>), (unsigned)count, kTileThreads, 0, s, a,
+                                   rp, first);
+            }
+            QG_HIP(hipGetLastError());
+            return;
+        }
+    }
     for (u64 first = 0; first < n_tiles; first += piece) {
         const u64 count = std::min(piece, n_tiles - first);
         hipLaunchKernelGGL((k_tile_pass<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, a, prog, first);
@@ -212,6 +427,7 @@ struct SweepOp {
 struct SweepPass {
     int m, low;
     int op_begin, n_ops;
+    int phase_begin, n_phases, n_slots;
     signed char pos[16];
 };
 
@@ -302,6 +518,165 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
         store_unit(phi, u, v);
         store_unit(lam, u, w);
     }
+}
+
+// Register-resident sweep pass (m == SweepBits): phi and lambda tiles in registers, one LDS
+// round trip per phase; each rotation's tile term is reduced over the wavefront at once and
+// over the workgroup at the end of the pass.
+constexpr int kMaxSweepSlots = 96;
+
+struct SweepPhase {
+    signed char rb[5], lb[6], wb[3];
+    short op_begin, n_ops;   // relative to the pass
+};
+
+template <class R, int A, int P>
+__device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const unsigned (&lidx)[A],
+                                               const Mat2<R> &ud, const Mat2<R> &gen, unsigned lc, double &s) {
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        if ((j >> P) & 1) continue;
+        constexpr int st = 1 << P;
+        if ((lidx[j] & lc) == lc) {
+            const Cx<R> p0 = vp[j], p1 = vp[j + st], l0 = vl[j], l1 = vl[j + st];
+            const Cx<R> q0 = cadd(cmul(gen.m00, p0), cmul(gen.m01, p1));
+            const Cx<R> q1 = cadd(cmul(gen.m10, p0), cmul(gen.m11, p1));
+            s += (double)l0.re * (double)q0.im - (double)l0.im * (double)q0.re + (double)l1.re * (double)q1.im -
+                 (double)l1.im * (double)q1.re;
+            vp[j] = cadd(cmul(ud.m00, p0), cmul(ud.m01, p1));
+            vp[j + st] = cadd(cmul(ud.m10, p0), cmul(ud.m11, p1));
+            vl[j] = cadd(cmul(ud.m00, l0), cmul(ud.m01, l1));
+            vl[j + st] = cadd(cmul(ud.m10, l0), cmul(ud.m11, l1));
+        }
+    }
+}
+
+template <class R, int A>
+__device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const unsigned (&lidx)[A],
+                                                  const Mat2<R> &ud, const Mat2<R> &gen, unsigned lc, int p,
+                                                  double &s) {
+    constexpr int RB = Log2<A>::v;
+    if constexpr (RB > 0) {
+        if (p == 0) reg_sweep_pair<R, A, 0>(vp, vl, lidx, ud, gen, lc, s);
+    }
+    if constexpr (RB > 1) {
+        if (p == 1) reg_sweep_pair<R, A, 1>(vp, vl, lidx, ud, gen, lc, s);
+    }
+    if constexpr (RB > 2) {
+        if (p == 2) reg_sweep_pair<R, A, 2>(vp, vl, lidx, ud, gen, lc, s);
+    }
+    if constexpr (RB > 3) {
+        if (p == 3) reg_sweep_pair<R, A, 3>(vp, vl, lidx, ud, gen, lc, s);
+    }
+}
+
+template <class R, int M, int T>
+__global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, Cx<R> *__restrict__ lam,
+                                                      const SweepPass pass, const SweepOp<R> *__restrict__ ops,
+                                                      const SweepPhase *__restrict__ phases,
+                                                      double *__restrict__ partial, u64 n_tiles, u64 first) {
+    constexpr int A = (1 << M) / T, RB = Log2<A>::v, WB = Log2<T / 64>::v, NW = T / 64;
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    __shared__ Cx<R> tp[1 << M];
+    __shared__ Cx<R> tl[1 << M];
+    __shared__ double red[kMaxSweepSlots][NW];
+    const u64 tile = first + blockIdx.x;
+    const u64 base = tile_base(tile, pass.pos, M);
+    constexpr int n_units = 1 << (M - lw);
+    for (int lu = threadIdx.x; lu < n_units; lu += T) {
+        const unsigned l = (unsigned)lu << lw;
+        const u64 u = (base | deposit(l, pass.pos, M, pass.low)) >> lw;
+        const Amps<R> x = load_unit(phi, u), y = load_unit(lam, u);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            tp[l + j] = x.v[j];
+            tl[l + j] = y.v[j];
+        }
+    }
+    __syncthreads();
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Cx<R> vp[A], vl[A];
+    unsigned lidx[A];
+    for (int ph = 0; ph < pass.n_phases; ++ph) {
+        const SweepPhase P = phases[pass.phase_begin + ph];
+        unsigned off = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb[k];
+#pragma unroll
+        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb[k];
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            unsigned l = off;
+#pragma unroll
+            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb[k];
+            lidx[j] = l;
+            vp[j] = tp[l];
+            vl[j] = tl[l];
+        }
+        for (int k = 0; k < P.n_ops; ++k) {
+            const SweepOp<R> &op = ops[pass.op_begin + P.op_begin + k];
+            const bool on = (base & op.gctrl) == op.gctrl;
+            const unsigned lc = op.lctrl;
+            double s = 0.;
+            if (on && op.kind == kOpPair) reg_sweep_pair_at<R, A>(vp, vl, lidx, op.udag, op.gen, lc, op.target, s);
+            if (on && op.kind != kOpPair) {
+                const bool outer = op.kind == kOpDiagOuter;
+                const bool obit = ((base >> op.target) & 1) != 0;
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    if ((lidx[j] & lc) != lc) continue;
+                    const bool one = outer ? obit : ((lidx[j] >> op.target) & 1u) != 0;
+                    const Cx<R> x = vp[j], y = vl[j];
+                    const Cx<R> q = cmul(one ? op.gen.m11 : op.gen.m00, x);
+                    s += (double)y.re * (double)q.im - (double)y.im * (double)q.re;
+                    const Cx<R> f = one ? op.udag.m11 : op.udag.m00;
+                    vp[j] = cmul(f, x);
+                    vl[j] = cmul(f, y);
+                }
+            }
+            if (op.slot >= 0) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+                if (lane == 0) red[op.slot][wave] = s;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            tp[lidx[j]] = vp[j];
+            tl[lidx[j]] = vl[j];
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < pass.n_slots; k += T) {
+        double t = 0.;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += red[k][w];
+        partial[(u64)k * n_tiles + tile] = t;
+    }
+    for (int lu = threadIdx.x; lu < n_units; lu += T) {
+        const unsigned l = (unsigned)lu << lw;
+        const u64 u = (base | deposit(l, pass.pos, M, pass.low)) >> lw;
+        Amps<R> x, y;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            x.v[j] = tp[l + j];
+            y.v[j] = tl[l + j];
+        }
+        store_unit(phi, u, x);
+        store_unit(lam, u, y);
+    }
+}
+
+// terms[slot_gate[r]] = sum over tiles of partial[r * n_tiles + tile], fixed order; one
+// wavefront per slot row.
+__global__ __launch_bounds__(64) void k_slot_rows(const double *__restrict__ partial, u64 n_tiles,
+                                                  const int *__restrict__ slot_gate, double *__restrict__ terms) {
+    const double *row = partial + (u64)blockIdx.x * n_tiles;
+    double s = 0.;
+    for (u64 i = threadIdx.x; i < n_tiles; i += 64) s += row[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (threadIdx.x == 0) terms[slot_gate[blockIdx.x]] = s;
 }
 
 // terms[slot] (+)= sum over tiles of partial[slot * n_tiles + tile], fixed order; one
@@ -446,10 +821,10 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     typedef SweepCfg<R> C;
     const int mb = std::min(max_bits > 0 ? std::min(max_bits, C::MaxBits) : C::MaxBits, n_lanes);
     const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
-    // plan the passes: like apply_gate_sequence, without a limit on gates per pass
+    // plan the passes: like apply_gate_sequence, with at most kMaxSweepSlots rotations a pass
     struct Plan {
         std::vector<int> bits;
-        int begin, end;
+        int begin, end, slots;
     };
     std::vector<Plan> plans;
     Plan cur;
@@ -457,6 +832,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         cur.bits.clear();
         for (int b = 0; b < low; ++b) cur.bits.push_back(b);
         cur.begin = cur.end = at;
+        cur.slots = 0;
     };
     auto has = [&](int b) { return std::find(cur.bits.begin(), cur.bits.end(), b) != cur.bits.end(); };
     open(0);
@@ -464,19 +840,25 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         const SweepGate &g = gates[i];
         const bool diag = is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3);
         const bool need = !(diag || has(g.target));
-        if ((int)cur.bits.size() + (need ? 1 : 0) > mb) {
+        if ((int)cur.bits.size() + (need ? 1 : 0) > mb || (g.pauli != 0 && cur.slots == kMaxSweepSlots)) {
             plans.push_back(cur);
             open(i);
         }
         if (need) cur.bits.push_back(g.target);
+        if (g.pauli != 0) ++cur.slots;
         cur.end = i + 1;
     }
     if (cur.end > cur.begin) plans.push_back(cur);
 
-    // device op array, pass by pass; slots numbered within each pass
-    std::vector<SweepOp<R>> hops(gates.size());
+    constexpr int A = (1 << C::MaxBits) / kTileThreads;
+    const bool regs = use_register_tiles();
+    // device tables: ops (pass by pass), phases of register passes, slot -> gate index
+    std::vector<SweepOp<R>> hops(std::max<size_t>(1, gates.size()));
+    std::vector<SweepPhase> hphases;
+    std::vector<int> slot_gate;
     std::vector<SweepPass> passes;
-    std::vector<std::vector<int>> slot_gate;   // per pass: gate index of each slot
+    std::vector<int> pass_slot_begin;
+    std::vector<bool> pass_regs;
     int max_slots = 0;
     for (const Plan &pl : plans) {
         std::vector<int> bits(pl.bits);
@@ -488,11 +870,14 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         ps.low = low;
         ps.op_begin = pl.begin;
         ps.n_ops = pl.end - pl.begin;
+        ps.phase_begin = (int)hphases.size();
+        ps.n_phases = 0;
         for (int j = 0; j < 16; ++j) ps.pos[j] = j < ps.m ? (signed char)bits[j] : (signed char)0;
         int local[64];
         for (int b = 0; b < 64; ++b) local[b] = -1;
         for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
-        std::vector<int> slots;
+        pass_slot_begin.push_back((int)slot_gate.size());
+        int slots = 0;
         for (int i = pl.begin; i < pl.end; ++i) {
             const SweepGate &g = gates[i];
             SweepOp<R> &op = hops[i];
@@ -517,43 +902,77 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
                 op.target = (short)g.target;
                 op.kind = (short)kOpDiagOuter;
             }
-            op.slot = g.pauli != 0 ? (int)slots.size() : -1;
-            if (g.pauli != 0) slots.push_back(i);
+            op.slot = g.pauli != 0 ? slots++ : -1;
+            if (g.pauli != 0) slot_gate.push_back(i);
         }
-        max_slots = std::max(max_slots, (int)slots.size());
+        ps.n_slots = slots;
+        const bool use_regs = regs && ps.m == C::MaxBits;
+        if (use_regs) {
+            std::vector<int> pair(ps.n_ops);
+            for (int k = 0; k < ps.n_ops; ++k)
+                pair[k] = hops[pl.begin + k].kind == kOpPair ? hops[pl.begin + k].target : -1;
+            const std::vector<PhaseLayout> L = plan_phase_layouts(pair, ps.m, Log2<A>::v, Log2<kTileThreads / 64>::v);
+            for (const PhaseLayout &P : L) {
+                SweepPhase h;
+                std::memset(&h, 0, sizeof(h));
+                for (int b = 0; b < (int)P.regs.size(); ++b) h.rb[b] = (signed char)P.regs[b];
+                for (int b = 0; b < 6; ++b) h.lb[b] = (signed char)P.lanes[b];
+                for (int b = 0; b < (int)P.waves.size(); ++b) h.wb[b] = (signed char)P.waves[b];
+                h.op_begin = (short)P.begin;
+                h.n_ops = (short)(P.end - P.begin);
+                for (int k = P.begin; k < P.end; ++k)
+                    if (hops[pl.begin + k].kind == kOpPair)
+                        hops[pl.begin + k].target = (short)reg_position(P, hops[pl.begin + k].target);
+                hphases.push_back(h);
+            }
+            ps.n_phases = (int)L.size();
+        }
+        pass_regs.push_back(use_regs);
+        max_slots = std::max(max_slots, slots);
         passes.push_back(ps);
-        slot_gate.push_back(slots);
     }
     if (passes.empty()) return;
+    if (hphases.empty()) hphases.push_back(SweepPhase());
+    if (slot_gate.empty()) slot_gate.push_back(0);
 
     hipStream_t s = stream();
-    const size_t ops_bytes = sizeof(SweepOp<R>) * hops.size();
-    SweepOp<R> *d_ops = static_cast<SweepOp<R> *>(dev_alloc(ops_bytes));
-    size_t part_bytes = 0;
+    size_t part_bytes = 8;
     for (const SweepPass &ps : passes)
         part_bytes = std::max(part_bytes, sizeof(double) * (size_t)max_slots * (size_t)(u64(1) << (n_lanes - ps.m)));
-    double *d_part = static_cast<double *>(dev_alloc(std::max<size_t>(part_bytes, 8)));
+    const size_t ops_bytes = sizeof(SweepOp<R>) * hops.size();
+    const size_t ph_bytes = sizeof(SweepPhase) * hphases.size();
+    const size_t sg_bytes = sizeof(int) * slot_gate.size();
+    const size_t o_ph = (ops_bytes + 255) & ~size_t(255), o_sg = (o_ph + ph_bytes + 255) & ~size_t(255);
+    const size_t o_part = (o_sg + sg_bytes + 255) & ~size_t(255);
+    const size_t total = o_part + part_bytes;
+    char *buf = static_cast<char *>(dev_alloc(total));
+    SweepOp<R> *d_ops = reinterpret_cast<SweepOp<R> *>(buf);
+    SweepPhase *d_ph = reinterpret_cast<SweepPhase *>(buf + o_ph);
+    int *d_sg = reinterpret_cast<int *>(buf + o_sg);
+    double *d_part = reinterpret_cast<double *>(buf + o_part);
     hipError_t e = hipMemcpyAsync(d_ops, hops.data(), ops_bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ph, hphases.data(), ph_bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sg, slot_gate.data(), sg_bytes, hipMemcpyHostToDevice, s);
     for (size_t pi = 0; pi < passes.size() && e == hipSuccess; ++pi) {
         const SweepPass &ps = passes[pi];
         const u64 n_tiles = u64(1) << (n_lanes - ps.m);
         const u64 piece = launch_piece_items() / kTileThreads;
         for (u64 first = 0; first < n_tiles; first += piece) {
             const u64 count = std::min(piece, n_tiles - first);
-            hipLaunchKernelGGL((k_tile_sweep<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
-                               d_ops, d_part, n_tiles, first);
+            if (pass_regs[pi])
+                hipLaunchKernelGGL((k_tile_sweep_reg<R, C::MaxBits, kTileThreads>), (unsigned)count, kTileThreads, 0, s,
+                                   phi, lam, ps, d_ops, d_ph, d_part, n_tiles, first);
+            else
+                hipLaunchKernelGGL((k_tile_sweep<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
+                                   d_ops, d_part, n_tiles, first);
         }
-        const std::vector<int> &slots = slot_gate[pi];
-        if (!slots.empty()) {
-            // terms of this pass's slots land at d_terms[gate index]
-            for (size_t k = 0; k < slots.size(); ++k)
-                hipLaunchKernelGGL(k_sum_slots, 1, 256, 0, s, d_part + k * n_tiles, n_tiles, d_terms + slots[k]);
-        }
+        if (ps.n_slots > 0)   // terms of this pass's slots land at d_terms[gate index]
+            hipLaunchKernelGGL(k_slot_rows, (unsigned)ps.n_slots, 64, 0, s, d_part, n_tiles, d_sg + pass_slot_begin[pi],
+                               d_terms);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(s);   // hops is a host temporary
-    dev_free(d_part, std::max<size_t>(part_bytes, 8));
-    dev_free(d_ops, ops_bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // the tables are host temporaries
+    dev_free(buf, total);
     QG_HIP(e);
 }
 

@@ -62,6 +62,13 @@ u64 g_piece_items = u64(1) << 30;
 
 u64 launch_piece_items() { return g_piece_items; }
 
+namespace {
+bool g_register_tiles = true;
+}
+
+bool use_register_tiles() { return g_register_tiles; }
+void set_register_tiles(bool on) { g_register_tiles = on; }
+
 void set_launch_piece_items(u64 items) {
     QG_REQUIRE(items >= (u64(1) << 12) && items <= (u64(1) << 30) && (items & (items - 1)) == 0,
                "launch piece must be a power of two in [2^12, 2^30]");

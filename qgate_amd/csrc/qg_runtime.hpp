@@ -59,4 +59,8 @@ void for_each_piece(u64 n_items, F &&f) {
 
 inline unsigned grid_for(u64 count, int block) { return (unsigned)((count + block - 1) / block); }
 
+// Full-size tile passes keep the tile in registers (default) or work on it in LDS.
+bool use_register_tiles();
+void set_register_tiles(bool on);
+
 }  // namespace qg

@@ -56,6 +56,7 @@ def lib():
         'qghip_devices_clear': (ctypes.c_int, []),
         'qghip_device_info': (ctypes.c_int, [_c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_uint64)]),
         'qghip_set_launch_piece': (ctypes.c_int, [ctypes.c_int64]),
+        'qghip_set_register_tiles': (ctypes.c_int, [ctypes.c_int]),
         'qghip_register_matrix_factory': (ctypes.c_int, [ctypes.c_char_p, _c_int_p]),
         'qghip_gate_matrix': (ctypes.c_int, [ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_int, _c_double_p]),
         'qghip_qubit_states_new': (ctypes.c_int, [ctypes.c_int, _c_void_pp]),
@@ -183,6 +184,11 @@ def device_info():
     dev, ncu, mem = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
     check(lib().qghip_device_info(ctypes.byref(dev), ctypes.byref(ncu), ctypes.byref(mem)))
     return {'device_id': dev.value, 'n_cus': ncu.value, 'total_mem': mem.value}
+
+
+def set_register_tiles(on):
+    """Full-size tile passes in registers (True, default) or in LDS (False)."""
+    check(lib().qghip_set_register_tiles(1 if on else 0))
 
 
 def set_launch_piece(items):

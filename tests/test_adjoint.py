@@ -135,17 +135,23 @@ def test_mixed_gradient_vs_central_differences(hip_module, port_module, seed, n)
 @pytest.mark.gpu
 @pytest.mark.parametrize('dtype', DTYPES)
 @pytest.mark.parametrize('n', [5, 12, 15])
-def test_tile_passes_match_per_gate_launches(hip_module, dtype, n):
-    """The tile-pass forward and backward sweep against one launch per gate and per step."""
+@pytest.mark.parametrize('registers', [True, False])
+def test_tile_passes_match_per_gate_launches(hip_module, dtype, n, registers):
+    """The tile-pass forward and backward sweep (full-size tiles in registers or in LDS)
+    against one launch per gate and per step."""
     spec = ac.mixed_spec(n, 120, seed=n, n_params=30)
     h = ac.random_observable(n, 12, seed=n + 1)
     params = np.random.RandomState(n + 2).uniform(-math.pi, math.pi, spec.n_params)
     out = {}
-    for mode in ('off', 'tile'):
-        circ = spec.build(dtype).set_fusion(mode)
-        value, grad = circ.gradient(params, h)
-        circ.expectation(params, h)
-        out[mode] = (value, grad, circ.state())
+    hip_module.set_register_tiles(registers)
+    try:
+        for mode in ('off', 'tile'):
+            circ = spec.build(dtype).set_fusion(mode)
+            value, grad = circ.gradient(params, h)
+            circ.expectation(params, h)
+            out[mode] = (value, grad, circ.state())
+    finally:
+        hip_module.set_register_tiles(True)
     tol = 1e-12 if dtype == np.float64 else 2e-4
     assert abs(out['off'][0] - out['tile'][0]) < tol
     assert np.max(np.abs(out['off'][1] - out['tile'][1])) < tol

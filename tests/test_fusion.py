@@ -159,3 +159,21 @@ def test_fusion_mode_checked(hip_module):
         hip_module.set_fusion('dense')
     with pytest.raises(ValueError):
         hip_module.create_qubit_states(np.float64).processor.set_fusion('tile', 20, -1)
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('registers', [True, False])
+def test_register_and_lds_tiles(hip_module, port_module, dtype, registers):
+    """Full-size tiles in registers (phases of up to 4 / 3 register-bit targets) and in LDS."""
+    n = 15
+    q, ops = _mixed_circuit(n, 500, seed=77)
+    ref = _sim(port_module, ops, q, 'one_static', np.float64)
+    hip_module.set_register_tiles(registers)
+    try:
+        with _Fusion(hip_module, 'tile'):
+            got = _sim(hip_module, ops, q, 'one_static', dtype)
+            err = max_abs(got.qubits.states[:], ref.qubits.states[:])
+    finally:
+        hip_module.set_register_tiles(True)
+    print('registers', registers, dtype.__name__, err)
+    assert err < tol(dtype), err
