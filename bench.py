@@ -16,6 +16,9 @@ HIP events on the library stream over every target bit; achieved = 2 S 2^30 byte
 launch time, against 8 TB/s; traffic = HBM bytes per launch measured with the FETCH_SIZE and
 WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md).
 
+large_state_34q: a 34-qubit complex64 state (128 GiB): per-target gate GB/s split at target 7,
+P(0) throughput and a 30-of-34-qubit sampling-pool build (prefix sums over 2^34 amplitudes).
+
 batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), each with its
 own parameters and observable, <H> and gradients in one graph replay, against a loop of
 single-circuit evaluations.
@@ -236,6 +239,52 @@ def batch_timing(n_circuits=4096, layers=10, seed=5, loop_sample=512):
             'groups': [{k: g[k] for k in ('qubits', 'items', 'tile_bits', 'threads', 'grid_per_cu')} for g in tuning]}
 
 
+def large_state_timing(n=34):
+    """Config of the large single-GPU state: 34 qubits complex64 (128 GiB). H on every qubit,
+    then per-target timing of an uncontrolled gate (HIP events, 2 launches per target), P(0)
+    of every qubit, and a sampling pool over 30 qubits (4 hidden) as prefix-sum throughput."""
+    from qgate_amd.simulator import hipruntime
+    from qgate_amd.simulator.lanes import Lane
+    info = hipruntime.device_info()
+    amp = 8
+    if info['total_mem'] < (amp << n) + (40 << 30):
+        return {'skipped': 'device memory {} GB'.format(info['total_mem'] // (1 << 30))}
+    r = Runner(hipruntime, n, np.float32)
+    t0 = time.perf_counter()
+    for q in range(n):
+        r.proc.apply_gate(r.H, False, r.qs, q)
+    r.sync()
+    h_s = time.perf_counter() - t0
+    mat = hipruntime.gate_matrix(r.gt.U(0.3, 1.1, 2.3))
+    per_target = [r.proc.profile_gate(mat, r.qs, [], t, 2) for t in range(n)]
+    launch_bytes = 2 * amp * (1 << n)
+    gbs = [launch_bytes / (ms * 1e-3) / 1e9 for ms in per_target]
+    t0 = time.perf_counter()
+    probs = [r.proc.calc_probability(r.qs, q) for q in range(n)]
+    prob_s = (time.perf_counter() - t0) / n
+    getter = hipruntime.create_qubits_states_getter(np.float32)
+    lanes = []
+    for i in range(n):
+        ln = Lane(None, i)
+        ln.set_external(i if i < 30 else -1)
+        lanes.append(ln)
+    t0 = time.perf_counter()
+    pool = getter.create_sampling_pool(list(range(30)), 30, n - 30, [(r.qs, lanes)], [])
+    pool_s = time.perf_counter() - t0
+    pool.delete()
+    r.delete()
+    return {'qubits': n, 'dtype': 'complex64', 'state_GiB': (amp << n) >> 30,
+            'h_all_ms': round(h_s * 1e3, 1),
+            'gate_GBs_targets_lt7': round(float(np.mean(gbs[:7])), 1),
+            'gate_GBs_targets_ge7': round(float(np.mean(gbs[7:])), 1),
+            'per_target_GBs': [round(v) for v in gbs],
+            'prob_ms_per_qubit': round(prob_s * 1e3, 2),
+            'prob_GBs': round((amp << (n - 1)) / prob_s / 1e9, 1),
+            'min_prob0_after_h': round(min(probs), 6), 'max_prob0_after_h': round(max(probs), 6),
+            'pool_build_30_of_34_ms': round(pool_s * 1e3, 1),
+            'pool_build_GBs': round((amp << n) / pool_s / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -245,6 +294,7 @@ def main():
     ap.add_argument('--gates', type=int, default=2000)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-batch', action='store_true')
+    ap.add_argument('--no-large', action='store_true')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the per-target gate timing loop and a 2 GiB copy kernel (for counter runs)')
     args = ap.parse_args()
@@ -328,6 +378,8 @@ def main():
         print('[bench] adjoint 24q {}: {}'.format(name, adjoint[name]), file=sys.stderr, flush=True)
     batched = batch_timing() if not args.no_batch else None
     print('[bench] batch: {}'.format(batched), file=sys.stderr, flush=True)
+    large = large_state_timing() if not args.no_large else None
+    print('[bench] large state: {}'.format(large), file=sys.stderr, flush=True)
 
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'GB/s', 'n_gpus': 1,
@@ -348,6 +400,7 @@ def main():
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
         'adjoint_24q': adjoint,
         'batched_adjoint': batched,
+        'large_state_34q': large,
         'roofline': roof,
     }
     if not args.no_cpu:

@@ -117,29 +117,14 @@ __global__ __launch_bounds__(256) void k_batch_zero(Cx<R> *a, u64 n_units, int u
     store_unit(a, u, v);
 }
 
-template <class R>
-__device__ __forceinline__ void load_tile(Cx<R> *t, const Cx<R> *a, u64 base, const BPass &ps, int threads) {
-    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
-    const int n_units = 1 << (ps.m - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += threads) {
-        const unsigned l = (unsigned)lu << lw;
-        const Amps<R> v = load_unit(a, (base | deposit(l, ps.pos, ps.m, ps.low)) >> lw);
-#pragma unroll
-        for (int j = 0; j < W; ++j) t[l + j] = v.v[j];
-    }
+template <class R, int T>
+__device__ __forceinline__ void load_tile(Cx<R> *t, const Cx<R> *a, u64 base, const BPass &ps) {
+    tile_to_lds<R, T, 8>(t, a, base, ps.pos, ps.m, ps.low);
 }
 
-template <class R>
-__device__ __forceinline__ void store_tile(Cx<R> *a, const Cx<R> *t, u64 base, const BPass &ps, int threads) {
-    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
-    const int n_units = 1 << (ps.m - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += threads) {
-        const unsigned l = (unsigned)lu << lw;
-        Amps<R> v;
-#pragma unroll
-        for (int j = 0; j < W; ++j) v.v[j] = t[l + j];
-        store_unit(a, (base | deposit(l, ps.pos, ps.m, ps.low)) >> lw, v);
-    }
+template <class R, int T>
+__device__ __forceinline__ void store_tile(Cx<R> *a, const Cx<R> *t, u64 base, const BPass &ps) {
+    lds_to_tile<R, T, 8>(a, t, base, ps.pos, ps.m, ps.low);
 }
 
 // Forward pass: the pass's gates on every tile, each with its item's matrix.
@@ -151,7 +136,7 @@ __global__ __launch_bounds__(T) void k_batch_pass(Cx<R> *__restrict__ a, const B
     for (u64 w = blockIdx.x; w < n_tiles; w += gridDim.x) {
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
-        load_tile(t, a, base, ps, T);
+        load_tile<R, T>(t, a, base, ps);
         __syncthreads();
         for (int k = 0; k < ps.n_ops; ++k) {
             const BOp op = ops[ps.op_begin + k];
@@ -181,7 +166,7 @@ __global__ __launch_bounds__(T) void k_batch_pass(Cx<R> *__restrict__ a, const B
             }
             __syncthreads();
         }
-        store_tile(a, t, base, ps, T);
+        store_tile<R, T>(a, t, base, ps);
         __syncthreads();
     }
 }
@@ -202,18 +187,21 @@ __global__ __launch_bounds__(T) void k_batch_pauli(const Cx<R> *__restrict__ psi
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
         const double *c = coefs + item * (u64)n_terms * 2;
-        load_tile(tp, psi, base, ps, T);
+        load_tile<R, T>(tp, psi, base, ps);
         __syncthreads();
         double dot = 0.;
+        const u64 dt = base | deposit((unsigned)threadIdx.x << lw, ps.pos, ps.m, ps.low);
         for (int lu = threadIdx.x; lu < n_units; lu += T) {
             const unsigned l = (unsigned)lu << lw;
-            const u64 u = (base | deposit(l, ps.pos, ps.m, ps.low)) >> lw;
+            // state index of amplitude l: per-thread part | per-iteration part (disjoint bits)
+            const u64 gl = dt | deposit((unsigned)(lu - (int)threadIdx.x) << lw, ps.pos, ps.m, ps.low);
+            const u64 u = gl >> lw;
             Amps<R> out;
             if (ps.accumulate) out = load_unit(lam, u);
 #pragma unroll
             for (int j = 0; j < W; ++j) {
                 const unsigned x = l + j;
-                const u64 gx = base | deposit(x, ps.pos, ps.m, ps.low);
+                const u64 gx = gl | (u64)j;   // bit 0 is local bit 0 whenever W == 2
                 double ar = ps.accumulate ? (double)out.v[j].re : 0., ai = ps.accumulate ? (double)out.v[j].im : 0.;
                 for (int g = 0; g < ps.n_ops; ++g) {
                     const PauliGroup gr = groups[ps.op_begin + g];
@@ -255,8 +243,8 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
         const u64 tin = w % tiles_per_item;
-        load_tile(tp, phi, base, ps, T);
-        load_tile(tl, lam, base, ps, T);
+        load_tile<R, T>(tp, phi, base, ps);
+        load_tile<R, T>(tl, lam, base, ps);
         __syncthreads();
         for (int k = 0; k < ps.n_ops; ++k) {
             const BOp op = ops[ps.op_begin + k];
@@ -310,8 +298,8 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
             }
             __syncthreads();
         }
-        store_tile(phi, tp, base, ps, T);
-        store_tile(lam, tl, base, ps, T);
+        store_tile<R, T>(phi, tp, base, ps);
+        store_tile<R, T>(lam, tl, base, ps);
         __syncthreads();
     }
 }

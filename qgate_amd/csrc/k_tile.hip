@@ -14,6 +14,7 @@
 // The gate list of a pass travels in the kernel arguments (at most 40 / 24 gates).
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -73,13 +74,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
     constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
     const int m = prog.m;
     const u64 base = tile_base(first + blockIdx.x, prog.pos, m);
-    const int n_units = 1 << (m - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
-        const unsigned l = (unsigned)lu << lw;
-        const Amps<R> v = load_unit(a, (base | deposit(l, prog.pos, m, prog.low)) >> lw);
-#pragma unroll
-        for (int j = 0; j < W; ++j) t[l + j] = v.v[j];
-    }
+    tile_to_lds<R, kTileThreads>(t, a, base, prog.pos, m, prog.low);
     __syncthreads();
     const int n_amps = 1 << m;
     for (int k = 0; k < prog.n_ops; ++k) {
@@ -108,13 +103,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
         }
         __syncthreads();
     }
-    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
-        const unsigned l = (unsigned)lu << lw;
-        Amps<R> v;
-#pragma unroll
-        for (int j = 0; j < W; ++j) v.v[j] = t[l + j];
-        store_unit(a, (base | deposit(l, prog.pos, m, prog.low)) >> lw, v);
-    }
+    lds_to_tile<R, kTileThreads>(a, t, base, prog.pos, m, prog.low);
 }
 
 bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
@@ -154,40 +143,63 @@ struct Log2<1> {
     static constexpr int v = 0;
 };
 
-template <class R, int A, int P>
-__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc) {
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-        if ((j >> P) & 1) continue;
-        constexpr int step = 1 << P;
-        if ((lidx[j] & lc) == lc) {
-            const Cx<R> x0 = v[j], x1 = v[j + step];
-            v[j] = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
-            v[j + step] = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
-        }
+// f(std::integral_constant<int, J>) for J = B .. E-1: register arrays are only ever indexed
+// with compile-time constants, so they stay in registers.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>());
+        static_for<B + 1, E>(f);
     }
 }
 
-template <class R, int A>
+// Pair update on register bit P. F = 0: general matrix; 1: swap; 2: real matrix.
+template <class R, int A, int P, int F>
+__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (((j >> P) & 1) == 0) {
+            constexpr int j1 = j + (1 << P);
+            if ((lidx[j] & lc) == lc) {
+                const Cx<R> x0 = v[j], x1 = v[j1];
+                if constexpr (F == 1) {
+                    v[j] = x1;
+                    v[j1] = x0;
+                } else if constexpr (F == 2) {
+                    const R a = mm.m00.re, b = mm.m01.re, c = mm.m10.re, d = mm.m11.re;
+                    v[j] = cx<R>(a * x0.re + b * x1.re, a * x0.im + b * x1.im);
+                    v[j1] = cx<R>(c * x0.re + d * x1.re, c * x0.im + d * x1.im);
+                } else {
+                    v[j] = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
+                    v[j1] = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
+                }
+            }
+        }
+    });
+}
+
+template <class R, int A, int F>
 __device__ __forceinline__ void reg_pair_at(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc,
                                             int p) {
     constexpr int RB = Log2<A>::v;
     if constexpr (RB > 0) {
-        if (p == 0) reg_pair<R, A, 0>(v, lidx, mm, lc);
+        if (p == 0) reg_pair<R, A, 0, F>(v, lidx, mm, lc);
     }
     if constexpr (RB > 1) {
-        if (p == 1) reg_pair<R, A, 1>(v, lidx, mm, lc);
+        if (p == 1) reg_pair<R, A, 1, F>(v, lidx, mm, lc);
     }
     if constexpr (RB > 2) {
-        if (p == 2) reg_pair<R, A, 2>(v, lidx, mm, lc);
+        if (p == 2) reg_pair<R, A, 2, F>(v, lidx, mm, lc);
     }
     if constexpr (RB > 3) {
-        if (p == 3) reg_pair<R, A, 3>(v, lidx, mm, lc);
+        if (p == 3) reg_pair<R, A, 3, F>(v, lidx, mm, lc);
     }
     if constexpr (RB > 4) {
-        if (p == 4) reg_pair<R, A, 4>(v, lidx, mm, lc);
+        if (p == 4) reg_pair<R, A, 4, F>(v, lidx, mm, lc);
     }
 }
+
+__host__ __device__ constexpr bool is_pair_kind(int k) { return k == kOpPair || k == kOpPairSwap || k == kOpPairReal; }
 
 template <class R, int M, int T>
 __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const RegProgram<R> prog, u64 first) {
@@ -195,13 +207,7 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
     constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
     __shared__ Cx<R> t[1 << M];
     const u64 base = tile_base(first + blockIdx.x, prog.pos, M);
-    constexpr int n_units = 1 << (M - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += T) {
-        const unsigned l = (unsigned)lu << lw;
-        const Amps<R> x = load_unit(a, (base | deposit(l, prog.pos, M, prog.low)) >> lw);
-#pragma unroll
-        for (int j = 0; j < W; ++j) t[l + j] = x.v[j];
-    }
+    tile_to_lds<R, T, (1 << (M - lw)) / T>(t, a, base, prog.pos, M, prog.low);
     __syncthreads();
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Cx<R> v[A];
@@ -213,42 +219,53 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
         for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb[k];
 #pragma unroll
         for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb[k];
-#pragma unroll
-        for (int j = 0; j < A; ++j) {
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
             unsigned l = off;
 #pragma unroll
             for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb[k];
             lidx[j] = l;
             v[j] = t[l];
-        }
+        });
         for (int k = 0; k < P.n_ops; ++k) {
-            const TileOp<R> &op = prog.ops[P.op_begin + k];
-            if ((base & op.gctrl) != op.gctrl) continue;   // same for the whole workgroup
-            const unsigned lc = op.lctrl;
-            if (op.kind == kOpPair) reg_pair_at<R, A>(v, lidx, op.m, lc, op.target);
-            if (op.kind == kOpDiagLocal) {
-#pragma unroll
-                for (int j = 0; j < A; ++j)
-                    if ((lidx[j] & lc) == lc) v[j] = cmul(((lidx[j] >> op.target) & 1u) ? op.m.m11 : op.m.m00, v[j]);
+            // plain locals: the lambdas below must not capture the kernel argument by address
+            const u64 gctrl = prog.ops[P.op_begin + k].gctrl;
+            if ((base & gctrl) != gctrl) continue;   // same for the whole workgroup
+            const Mat2<R> mm = prog.ops[P.op_begin + k].m;
+            const unsigned lc = prog.ops[P.op_begin + k].lctrl;
+            const int tgt = prog.ops[P.op_begin + k].target;
+            const int kind = prog.ops[P.op_begin + k].kind;
+            if (kind == kOpPair) reg_pair_at<R, A, 0>(v, lidx, mm, lc, tgt);
+            if (kind == kOpPairSwap) reg_pair_at<R, A, 1>(v, lidx, mm, lc, tgt);
+            if (kind == kOpPairReal) reg_pair_at<R, A, 2>(v, lidx, mm, lc, tgt);
+            if (kind == kOpDiagLocal) {
+                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    if ((lidx[j] & lc) == lc) v[j] = cmul(((lidx[j] >> tgt) & 1u) ? mm.m11 : mm.m00, v[j]);
+                });
             }
-            if (op.kind == kOpDiagOuter) {
-                const Cx<R> f = ((base >> op.target) & 1) ? op.m.m11 : op.m.m00;
-#pragma unroll
-                for (int j = 0; j < A; ++j)
+            if (kind == kOpDiagPhase) {
+                const unsigned need = lc | (1u << tgt);
+                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    if ((lidx[j] & need) == need) v[j] = cmul(mm.m11, v[j]);
+                });
+            }
+            if (kind == kOpDiagOuter) {
+                const Cx<R> f = ((base >> tgt) & 1) ? mm.m11 : mm.m00;
+                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
                     if ((lidx[j] & lc) == lc) v[j] = cmul(f, v[j]);
+                });
             }
         }
-#pragma unroll
-        for (int j = 0; j < A; ++j) t[lidx[j]] = v[j];
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            t[lidx[j]] = v[j];
+        });
         __syncthreads();
     }
-    for (int lu = threadIdx.x; lu < n_units; lu += T) {
-        const unsigned l = (unsigned)lu << lw;
-        Amps<R> x;
-#pragma unroll
-        for (int j = 0; j < W; ++j) x.v[j] = t[l + j];
-        store_unit(a, (base | deposit(l, prog.pos, M, prog.low)) >> lw, x);
-    }
+    lds_to_tile<R, T, (1 << (M - lw)) / T>(a, t, base, prog.pos, M, prog.low);
 }
 
 // Phases of a pass: consecutive gates whose pair targets (pair[k] >= 0: local bit of a
@@ -298,7 +315,7 @@ int reg_position(const PhaseLayout &P, int bit) {
 template <class R>
 bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
     std::vector<int> pair(prog.n_ops);
-    for (int k = 0; k < prog.n_ops; ++k) pair[k] = prog.ops[k].kind == kOpPair ? prog.ops[k].target : -1;
+    for (int k = 0; k < prog.n_ops; ++k) pair[k] = is_pair_kind(prog.ops[k].kind) ? prog.ops[k].target : -1;
     const std::vector<PhaseLayout> L = plan_phase_layouts(pair, m, rb_count, wb_count);
     if ((int)L.size() > kMaxPhases) return false;
     prog.n_phases = (int)L.size();
@@ -311,7 +328,7 @@ bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
         P.op_begin = (unsigned char)L[i].begin;
         P.n_ops = (unsigned char)(L[i].end - L[i].begin);
         for (int k = L[i].begin; k < L[i].end; ++k)
-            if (prog.ops[k].kind == kOpPair) prog.ops[k].target = (short)reg_position(L[i], prog.ops[k].target);
+            if (is_pair_kind(prog.ops[k].kind)) prog.ops[k].target = (short)reg_position(L[i], prog.ops[k].target);
     }
     return true;
 }
@@ -392,7 +409,20 @@ void launch_pass(Cx<R> *a, int n_lanes, const PassPlan &plan, int low, int max_b
         rp.n_ops = prog.n_ops;
         rp.low = prog.low;
         std::memcpy(rp.pos, prog.pos, sizeof(rp.pos));
-        for (int k = 0; k < prog.n_ops; ++k) rp.ops[k] = prog.ops[k];
+        for (int k = 0; k < prog.n_ops; ++k) {
+            rp.ops[k] = prog.ops[k];
+            TileOp<R> &op = rp.ops[k];
+            const Mat2<R> &mm = op.m;
+            const R zero = R(0), one = R(1);
+            if (op.kind == kOpPair) {
+                const bool swap = mm.m00.re == zero && mm.m00.im == zero && mm.m11.re == zero && mm.m11.im == zero &&
+                                  mm.m01.re == one && mm.m01.im == zero && mm.m10.re == one && mm.m10.im == zero;
+                const bool real = mm.m00.im == zero && mm.m01.im == zero && mm.m10.im == zero && mm.m11.im == zero;
+                op.kind = (short)(swap ? kOpPairSwap : real ? kOpPairReal : kOpPair);
+            } else if (op.kind == kOpDiagLocal && mm.m00.re == one && mm.m00.im == zero) {
+                op.kind = (short)kOpDiagPhase;
+            }
+        }
         constexpr int A = (1 << C::MaxBits) / kTileThreads;
         if (plan_phases<R>(rp, prog.m, Log2<A>::v, Log2<kTileThreads / 64>::v)) {
             for (u64 first = 0; first < n_tiles; first += piece) {
@@ -447,17 +477,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
     const int m = pass.m;
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, m);
-    const int n_units = 1 << (m - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
-        const unsigned l = (unsigned)lu << lw;
-        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
-        const Amps<R> v = load_unit(phi, u), w = load_unit(lam, u);
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            tp[l + j] = v.v[j];
-            tl[l + j] = w.v[j];
-        }
-    }
+    tile_to_lds<R, kTileThreads, 4>(tp, phi, base, pass.pos, m, pass.low);
+    tile_to_lds<R, kTileThreads, 4>(tl, lam, base, pass.pos, m, pass.low);
     __syncthreads();
     const int n_amps = 1 << m;
     for (int k = 0; k < pass.n_ops; ++k) {
@@ -506,18 +527,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
         }
         __syncthreads();
     }
-    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
-        const unsigned l = (unsigned)lu << lw;
-        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
-        Amps<R> v, w;
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            v.v[j] = tp[l + j];
-            w.v[j] = tl[l + j];
-        }
-        store_unit(phi, u, v);
-        store_unit(lam, u, w);
-    }
+    lds_to_tile<R, kTileThreads, 4>(phi, tp, base, pass.pos, m, pass.low);
+    lds_to_tile<R, kTileThreads, 4>(lam, tl, base, pass.pos, m, pass.low);
 }
 
 // Register-resident sweep pass (m == SweepBits): phi and lambda tiles in registers, one LDS
@@ -582,17 +593,9 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
     __shared__ double red[kMaxSweepSlots][NW];
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, M);
-    constexpr int n_units = 1 << (M - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += T) {
-        const unsigned l = (unsigned)lu << lw;
-        const u64 u = (base | deposit(l, pass.pos, M, pass.low)) >> lw;
-        const Amps<R> x = load_unit(phi, u), y = load_unit(lam, u);
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            tp[l + j] = x.v[j];
-            tl[l + j] = y.v[j];
-        }
-    }
+    constexpr int upt = (1 << (M - lw)) / T;   // units per thread
+    tile_to_lds<R, T, upt>(tp, phi, base, pass.pos, M, pass.low);
+    tile_to_lds<R, T, upt>(tl, lam, base, pass.pos, M, pass.low);
     __syncthreads();
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Cx<R> vp[A], vl[A];
@@ -653,18 +656,8 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
         for (int w = 0; w < NW; ++w) t += red[k][w];
         partial[(u64)k * n_tiles + tile] = t;
     }
-    for (int lu = threadIdx.x; lu < n_units; lu += T) {
-        const unsigned l = (unsigned)lu << lw;
-        const u64 u = (base | deposit(l, pass.pos, M, pass.low)) >> lw;
-        Amps<R> x, y;
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            x.v[j] = tp[l + j];
-            y.v[j] = tl[l + j];
-        }
-        store_unit(phi, u, x);
-        store_unit(lam, u, y);
-    }
+    lds_to_tile<R, T, upt>(phi, tp, base, pass.pos, M, pass.low);
+    lds_to_tile<R, T, upt>(lam, tl, base, pass.pos, M, pass.low);
 }
 
 // terms[slot_gate[r]] = sum over tiles of partial[r * n_tiles + tile], fixed order; one
@@ -729,23 +722,21 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pauli(const Cx<R> *__rest
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, m);
     const int n_units = 1 << (m - lw);
-    for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
-        const unsigned l = (unsigned)lu << lw;
-        const Amps<R> v = load_unit(psi, (base | deposit(l, pass.pos, m, pass.low)) >> lw);
-#pragma unroll
-        for (int j = 0; j < W; ++j) tp[l + j] = v.v[j];
-    }
+    tile_to_lds<R, kTileThreads>(tp, psi, base, pass.pos, m, pass.low);
     __syncthreads();
     double dot = 0.;
+    const u64 dt = base | deposit((unsigned)threadIdx.x << lw, pass.pos, m, pass.low);
     for (int lu = threadIdx.x; lu < n_units; lu += kTileThreads) {
         const unsigned l = (unsigned)lu << lw;
-        const u64 u = (base | deposit(l, pass.pos, m, pass.low)) >> lw;
+        // state index of amplitude l: per-thread part | per-iteration part (disjoint bits)
+        const u64 gl = dt | deposit((unsigned)(lu - (int)threadIdx.x) << lw, pass.pos, m, pass.low);
+        const u64 u = gl >> lw;
         Amps<R> out;
         if (pass.accumulate) out = load_unit(lam, u);
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             const unsigned x = l + j;
-            const u64 gx = base | deposit(x, pass.pos, m, pass.low);
+            const u64 gx = gl | (u64)j;   // bit 0 is local bit 0 whenever W == 2
             double ar = pass.accumulate ? (double)out.v[j].re : 0., ai = pass.accumulate ? (double)out.v[j].im : 0.;
             for (int g = 0; g < pass.n_groups; ++g) {
                 const PauliGroup gr = groups[pass.g_begin + g];

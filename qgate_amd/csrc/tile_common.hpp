@@ -9,6 +9,10 @@ namespace qg {
 // diagonal on a tile bit, or as a diagonal whose target lies outside the tile (one factor for
 // the whole tile).
 enum : int { kOpPair = 0, kOpDiagLocal = 1, kOpDiagOuter = 2 };
+// Cheaper forms of the same updates (register-resident passes): a pair gate that only swaps
+// (X, CX, CCX targets), a pair gate with a real matrix (H, RY), and a diagonal on a tile bit
+// with m00 == 1 (Z, S, T, U1, CZ, cu1: only the |1> half changes).
+enum : int { kOpPairSwap = 3, kOpPairReal = 4, kOpDiagPhase = 5 };
 
 // State index of the first amplitude of tile `tile`: zero bits inserted at the tile bits
 // pos[0..m) (ascending).
@@ -25,6 +29,60 @@ __device__ __forceinline__ u64 deposit(unsigned l, const signed char *pos, int m
     u64 g = l & ((1u << low) - 1);
     for (int j = low; j < m; ++j) g |= u64((l >> j) & 1u) << pos[j];
     return g;
+}
+
+// Tile <-> LDS transfers of a workgroup of T threads: every thread issues up to CHUNK 16-byte
+// loads (or stores) before touching LDS, so each workgroup keeps CHUNK x T accesses in flight
+// instead of waiting on one load at a time. Unit lu = c T + tid has local index
+// (c T + tid) << lw, whose two parts occupy disjoint bits, so its state offset is
+// deposit(tid << lw) | deposit(c T << lw): one deposit per thread and one per chunk (the
+// same for the whole workgroup) instead of one per access.
+template <class R, int T, int CHUNK = 8>
+__device__ __forceinline__ void tile_to_lds(Cx<R> *t, const Cx<R> *a, u64 base, const signed char *pos, int m,
+                                            int low) {
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const int n_units = 1 << (m - lw);
+    const u64 dt = base | deposit((unsigned)threadIdx.x << lw, pos, m, low);
+    for (int b0 = 0; b0 < n_units; b0 += T * CHUNK) {
+        Amps<R> x[CHUNK];
+#pragma unroll
+        for (int c = 0; c < CHUNK; ++c) {
+            const int lu = b0 + c * T + (int)threadIdx.x;
+            if (lu < n_units) x[c] = load_unit(a, (dt | deposit((unsigned)(b0 + c * T) << lw, pos, m, low)) >> lw);
+        }
+#pragma unroll
+        for (int c = 0; c < CHUNK; ++c) {
+            const int lu = b0 + c * T + (int)threadIdx.x;
+            if (lu < n_units) {
+#pragma unroll
+                for (int j = 0; j < W; ++j) t[((unsigned)lu << lw) + j] = x[c].v[j];
+            }
+        }
+    }
+}
+
+template <class R, int T, int CHUNK = 8>
+__device__ __forceinline__ void lds_to_tile(Cx<R> *a, const Cx<R> *t, u64 base, const signed char *pos, int m,
+                                            int low) {
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const int n_units = 1 << (m - lw);
+    const u64 dt = base | deposit((unsigned)threadIdx.x << lw, pos, m, low);
+    for (int b0 = 0; b0 < n_units; b0 += T * CHUNK) {
+        Amps<R> x[CHUNK];
+#pragma unroll
+        for (int c = 0; c < CHUNK; ++c) {
+            const int lu = b0 + c * T + (int)threadIdx.x;
+            if (lu < n_units) {
+#pragma unroll
+                for (int j = 0; j < W; ++j) x[c].v[j] = t[((unsigned)lu << lw) + j];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CHUNK; ++c) {
+            const int lu = b0 + c * T + (int)threadIdx.x;
+            if (lu < n_units) store_unit(a, (dt | deposit((unsigned)(b0 + c * T) << lw, pos, m, low)) >> lw, x[c]);
+        }
+    }
 }
 
 // Sum of v over a workgroup of B threads in a fixed order; valid in thread 0. scratch holds
