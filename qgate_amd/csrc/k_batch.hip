@@ -51,8 +51,8 @@ struct BMats {
 struct BOp {            // a gate inside a pass
     u64 gctrl;          // controls outside the tile (state bits of the item)
     unsigned lctrl;     // controls inside the tile (local bits)
-    short target;       // local bit, or the item's state bit for kOpDiagOuter
-    short kind;
+    int target;         // local bit, or the item's state bit for kOpDiagOuter
+    int kind;           // (dwords: a uniform dword field is one scalar load)
     int gate;           // index in the structure
     int pauli;          // generator of a rotation (sweep)
     int slot;           // gradient slot within the pass (sweep), -1 otherwise
@@ -63,7 +63,7 @@ struct BPass {
     int op_begin, n_ops;    // ops (gate passes) or groups (Pauli passes)
     int n_slots;            // sweep: gradient slots; Pauli: 1 on the last pass
     int accumulate;         // Pauli: add to lambda
-    signed char pos[16];
+    alignas(4) signed char pos[kTilePosMax];
 };
 
 struct BTermMask {
@@ -117,14 +117,14 @@ __global__ __launch_bounds__(256) void k_batch_zero(Cx<R> *a, u64 n_units, int u
     store_unit(a, u, v);
 }
 
-template <class R, int T>
+template <class R, int T, int MB>
 __device__ __forceinline__ void load_tile(Cx<R> *t, const Cx<R> *a, u64 base, const BPass &ps) {
-    tile_to_lds<R, T, 8>(t, a, base, ps.pos, ps.m, ps.low);
+    load_tile_lds<R, T, MB>(t, a, base, ps.pos, ps.m, ps.low);
 }
 
-template <class R, int T>
+template <class R, int T, int MB>
 __device__ __forceinline__ void store_tile(Cx<R> *a, const Cx<R> *t, u64 base, const BPass &ps) {
-    lds_to_tile<R, T, 8>(a, t, base, ps.pos, ps.m, ps.low);
+    store_tile_lds<R, T, MB>(a, t, base, ps.pos, ps.m, ps.low);
 }
 
 // Forward pass: the pass's gates on every tile, each with its item's matrix.
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(T) void k_batch_pass(Cx<R> *__restrict__ a, const B
     for (u64 w = blockIdx.x; w < n_tiles; w += gridDim.x) {
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
-        load_tile<R, T>(t, a, base, ps);
+        load_tile<R, T, MB>(t, a, base, ps);
         __syncthreads();
         for (int k = 0; k < ps.n_ops; ++k) {
             const BOp op = ops[ps.op_begin + k];
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(T) void k_batch_pass(Cx<R> *__restrict__ a, const B
             }
             __syncthreads();
         }
-        store_tile<R, T>(a, t, base, ps);
+        store_tile<R, T, MB>(a, t, base, ps);
         __syncthreads();
     }
 }
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(T) void k_batch_pauli(const Cx<R> *__restrict__ psi
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
         const double *c = coefs + item * (u64)n_terms * 2;
-        load_tile<R, T>(tp, psi, base, ps);
+        load_tile<R, T, MB>(tp, psi, base, ps);
         __syncthreads();
         double dot = 0.;
         const u64 dt = base | deposit((unsigned)threadIdx.x << lw, ps.pos, ps.m, ps.low);
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
         const u64 tin = w % tiles_per_item;
-        load_tile<R, T>(tp, phi, base, ps);
-        load_tile<R, T>(tl, lam, base, ps);
+        load_tile<R, T, MB>(tp, phi, base, ps);
+        load_tile<R, T, MB>(tl, lam, base, ps);
         __syncthreads();
         for (int k = 0; k < ps.n_ops; ++k) {
             const BOp op = ops[ps.op_begin + k];
@@ -298,8 +298,8 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
             }
             __syncthreads();
         }
-        store_tile<R, T>(phi, tp, base, ps);
-        store_tile<R, T>(lam, tl, base, ps);
+        store_tile<R, T, MB>(phi, tp, base, ps);
+        store_tile<R, T, MB>(lam, tl, base, ps);
         __syncthreads();
     }
 }

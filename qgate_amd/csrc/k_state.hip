@@ -160,6 +160,126 @@ __global__ __launch_bounds__(kWave) void k_reset(Cx<R> *__restrict__ a, BitInser
     store_unit(a, u0 | dist, z);
 }
 
+// A Pauli string P = (x) sigma_k over the lanes of x | z (X: x bit, Z: z bit, Y: both) acts
+// on a basis state as P|y> = c (-1)^|y & z| |y ^ x>, c = i^#Y, so
+//   (P psi)_y = c (-1)^|(y ^ x) & z| psi_(y ^ x).
+// The kernels below visit each pair (y0, y1 = y0 ^ x) once (y0 with bit h = top bit of x
+// clear) and use that P is Hermitian: ((1 + P) psi)_y0 and ((1 + P) psi)_y1 have equal moduli.
+//   mode 0: x == 0 (Z string): diagonal, every unit, each amplitude on its own;
+//   mode 1: complex64 with x == 1: the pair is the two amplitudes of one unit;
+//   mode 2: the pair spans two units u0, u1 = u0 ^ xu, amplitude j of u0 with j ^ xj of u1.
+struct PauliArgs {
+    u64 z;
+    u64 xu;           // x at unit level
+    BitInsert ins;    // zero bit at unit level h - lw (mode 2)
+    int xj;           // x & (W - 1)
+    int mode;
+    double cr, ci;    // c = cr + i ci
+};
+
+__device__ __forceinline__ double parity_sign(u64 y, u64 z) { return (__popcll(y & z) & 1) ? -1. : 1.; }
+
+// |a0 + c (-1)^|y1 & z| a1|^2 in FP64
+template <class R>
+__device__ __forceinline__ double pauli_pair_weight(const Cx<R> &a0, const Cx<R> &a1, u64 y1, const PauliArgs &P) {
+    const double sg = parity_sign(y1, P.z);
+    const double br = (double)a1.re, bi = (double)a1.im;
+    const double tr = (double)a0.re + sg * (P.cr * br - P.ci * bi);
+    const double ti = (double)a0.im + sg * (P.cr * bi + P.ci * br);
+    return tr * tr + ti * ti;
+}
+
+// Per-block sums of mode 0: |psi_y|^2 over even |y & z|; modes 1, 2: the pair weights above
+// (the host halves them).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pauli_prob_partial(const Cx<R> *__restrict__ a, const PauliArgs P,
+                                                               u64 n_items, double *partial) {
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    double s = 0.;
+    const u64 base = (u64)blockIdx.x * (kBlock * kProbUnitsPerThread) + threadIdx.x;
+#pragma unroll 4
+    for (int r = 0; r < kProbUnitsPerThread; ++r) {
+        const u64 w = base + (u64)r * kBlock;
+        if (w >= n_items) continue;
+        if (P.mode == 0) {
+            const Amps<R> v = load_unit(a, w);
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (parity_sign((w << lw) | j, P.z) > 0.) s += abs2d(v.v[j]);
+        } else if (P.mode == 1) {
+            const Amps<R> v = load_unit(a, w);
+            s += pauli_pair_weight(v.v[0], v.v[W - 1], (w << 1) | 1, P);
+        } else {
+            const u64 u0 = insert_zero_bits(w, P.ins), u1 = u0 ^ P.xu;
+            const Amps<R> v0 = load_unit(a, u0), v1 = load_unit(a, u1);
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const int jj = j ^ P.xj;
+                s += pauli_pair_weight(v0.v[j], jj ? v1.v[W - 1] : v1.v[0], (u1 << lw) | (u64)jj, P);
+            }
+        }
+    }
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// (cr + i ci) v for cr, ci in {-1, 0, 1}: exact
+template <class R>
+__device__ __forceinline__ Cx<R> unit_phase(double cr, double ci, const Cx<R> &v) {
+    const R r = (R)cr, i = (R)ci;
+    return cx<R>(r * v.re - i * v.im, r * v.im + i * v.re);
+}
+
+// Collapse onto the eigenspace P = (-1)^value: psi <- (1 + sgn P) psi / (2 sqrt(p)), with
+// half_norm = 1 / (2 sqrt(p)) and sgn = (-1)^value; mode 0 keeps amplitudes with
+// |y & z| = value (mod 2), scaled by norm = 1 / sqrt(p), and zeroes the rest.
+template <class R>
+__global__ __launch_bounds__(kWave) void k_pauli_decohere(Cx<R> *__restrict__ a, const PauliArgs P, int value,
+                                                          R norm, R half_norm, u64 n_items, u64 first) {
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
+    const u64 w = first + (u64)blockIdx.x * kWave + threadIdx.x;
+    if (w >= n_items) return;
+    const double sgn = value ? -1. : 1.;
+    if (P.mode == 0) {
+        Amps<R> v = load_unit(a, w);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const bool keep = (int)(__popcll(((w << lw) | j) & P.z) & 1) == value;
+            v.v[j] = keep ? cx<R>(v.v[j].re * norm, v.v[j].im * norm) : cx<R>(0, 0);
+        }
+        store_unit(a, w, v);
+        return;
+    }
+    if (P.mode == 1) {
+        Amps<R> v = load_unit(a, w);
+        const Cx<R> a0 = v.v[0], a1 = v.v[W - 1];
+        const double s1 = sgn * parity_sign((w << 1) | 1, P.z), s0 = sgn * parity_sign(w << 1, P.z);
+        const Cx<R> t0 = cadd(a0, unit_phase(s1 * P.cr, s1 * P.ci, a1));
+        const Cx<R> t1 = cadd(a1, unit_phase(s0 * P.cr, s0 * P.ci, a0));
+        v.v[0] = cx<R>(t0.re * half_norm, t0.im * half_norm);
+        v.v[W - 1] = cx<R>(t1.re * half_norm, t1.im * half_norm);
+        store_unit(a, w, v);
+        return;
+    }
+    const u64 u0 = insert_zero_bits(w, P.ins), u1 = u0 ^ P.xu;
+    const Amps<R> v0 = load_unit(a, u0), v1 = load_unit(a, u1);
+    Amps<R> n0, n1;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const int jj = j ^ P.xj;
+        const Cx<R> b0 = v0.v[j], b1 = jj ? v1.v[W - 1] : v1.v[0];
+        const double s1 = sgn * parity_sign((u1 << lw) | (u64)jj, P.z);
+        const double s0 = sgn * parity_sign((u0 << lw) | (u64)j, P.z);
+        const Cx<R> t0 = cadd(b0, unit_phase(s1 * P.cr, s1 * P.ci, b1));
+        const Cx<R> t1 = cadd(b1, unit_phase(s0 * P.cr, s0 * P.ci, b0));
+        n0.v[j] = cx<R>(t0.re * half_norm, t0.im * half_norm);
+        if (jj) n1.v[W - 1] = cx<R>(t1.re * half_norm, t1.im * half_norm);
+        else n1.v[0] = cx<R>(t1.re * half_norm, t1.im * half_norm);
+    }
+    store_unit(a, u0, n0);
+    store_unit(a, u1, n1);
+}
+
 template <class R>
 __global__ void k_set_basis1(Cx<R> *a, int value) {
     a[0] = value ? cx<R>(0, 0) : cx<R>(1, 0);
@@ -341,7 +461,86 @@ void launch_join(Cx<R> *dst, int dst_lanes, const std::vector<const Cx<R> *> &sr
     QG_HIP(hipGetLastError());
 }
 
+namespace {
+
+template <class R>
+PauliArgs pauli_args(int n_lanes, u64 x, u64 z, u64 *n_items) {
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
+    QG_REQUIRE(n_lanes >= 1 && n_lanes < 64, "lane count %d out of range", n_lanes);
+    const u64 lanes = (u64(1) << n_lanes) - 1;
+    QG_REQUIRE(((x | z) & ~lanes) == 0, "Pauli lanes outside the state vector");
+    QG_REQUIRE((x | z) != 0, "the Pauli string is the identity");
+    PauliArgs P;
+    P.z = z;
+    P.ins = no_bits();
+    P.xu = 0;
+    P.xj = 0;
+    switch (__builtin_popcountll(x & z) & 3) {   // c = i^#Y
+        case 0: P.cr = 1.; P.ci = 0.; break;
+        case 1: P.cr = 0.; P.ci = 1.; break;
+        case 2: P.cr = -1.; P.ci = 0.; break;
+        default: P.cr = 0.; P.ci = -1.; break;
+    }
+    const u64 n_units = (u64(1) << n_lanes) / W;
+    if (x == 0) {
+        P.mode = 0;
+        *n_items = n_units;
+    } else if (W == 2 && x == 1) {
+        P.mode = 1;
+        *n_items = n_units;
+    } else {
+        const int h = 63 - __builtin_clzll(x);
+        P.mode = 2;
+        P.ins = one_bit(h - lw);
+        P.xu = x >> lw;
+        P.xj = (int)(x & (W - 1));
+        *n_items = n_units / 2;
+    }
+    return P;
+}
+
+}  // namespace
+
+template <class R>
+double calc_pauli_prob(const Cx<R> *a, int n_lanes, u64 x, u64 z) {
+    Runtime &rt = runtime();
+    hipStream_t s = stream();
+    u64 n_items;
+    const PauliArgs P = pauli_args<R>(n_lanes, x, z, &n_items);
+    const u64 per_block = (u64)kBlock * kProbUnitsPerThread;
+    const u64 n_blocks = (n_items + per_block - 1) / per_block;
+    const size_t bytes = sizeof(double) * (size_t)(n_blocks + 1);
+    double *partial = static_cast<double *>(dev_alloc(bytes));
+    hipLaunchKernelGGL((k_pauli_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, P, n_items, partial);
+    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, partial, n_blocks, partial + n_blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rt.h_pinned, partial + n_blocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(partial, bytes);
+    QG_HIP(e);
+    // pair modes summed |((1 + P) psi)_y0|^2 once per pair: p = sum / 2 (exact)
+    return P.mode == 0 ? rt.h_pinned[0] : 0.5 * rt.h_pinned[0];
+}
+
+template <class R>
+void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, double prob) {
+    QG_REQUIRE(value == 0 || value == 1, "value must be 0 or 1");
+    u64 n_items;
+    const PauliArgs P = pauli_args<R>(n_lanes, x, z, &n_items);
+    const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
+    const R rnorm = (R)norm, half = (R)(0.5 * norm);
+    hipStream_t s = stream();
+    for_each_piece(n_items, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL((k_pauli_decohere<R>), grid_for(count, kWave), kWave, 0, s, a, P, value, rnorm, half,
+                           n_items, first);
+    });
+    QG_HIP(hipGetLastError());
+}
+
 #define QG_INST(R)                                                                              \
+    template double calc_pauli_prob<R>(const Cx<R> *, int, u64, u64);                         \
+    template void launch_pauli_decohere<R>(Cx<R> *, int, u64, u64, int, double);              \
     template void launch_set_zero_state<R>(Cx<R> *, int);                                     \
     template double calc_prob<R>(const Cx<R> *, int, int);                                    \
     template void launch_decohere<R>(Cx<R> *, int, int, int, R);                              \

@@ -56,7 +56,7 @@ struct TileProgram {
     int m;              // tile bits
     int low;            // pos[j] == j for j < low
     int n_ops;
-    signed char pos[16];   // state bit of local bit j, ascending
+    alignas(4) signed char pos[kTilePosMax];   // state bit of local bit j, ascending
     TileOp<R> ops[TileCfg<R>::MaxOps];
 };
 
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
     constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0;
     const int m = prog.m;
     const u64 base = tile_base(first + blockIdx.x, prog.pos, m);
-    tile_to_lds<R, kTileThreads>(t, a, base, prog.pos, m, prog.low);
+    load_tile_lds<R, kTileThreads, MB>(t, a, base, prog.pos, m, prog.low);
     __syncthreads();
     const int n_amps = 1 << m;
     for (int k = 0; k < prog.n_ops; ++k) {
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
         }
         __syncthreads();
     }
-    lds_to_tile<R, kTileThreads>(a, t, base, prog.pos, m, prog.low);
+    store_tile_lds<R, kTileThreads, MB>(a, t, base, prog.pos, m, prog.low);
 }
 
 bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
@@ -119,17 +119,28 @@ bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0
 
 constexpr int kMaxPhases = 40;
 
-struct RegPhase {
-    signed char rb[5];   // local bit of register bit k (k < RB)
-    signed char lb[6];   // local bit of lane bit k
-    signed char wb[3];   // local bit of wave bit k
-    unsigned char op_begin, n_ops;
+// Bit layout of one register phase, packed in dwords so that a phase read at a runtime
+// index takes scalar loads (a byte array read at a runtime index would be a vector load,
+// whose wait also waits for HBM traffic in flight). Byte k of `bits` is the local bit of
+// register bit k (k < 5), of lane bit k - 5 (5 <= k < 11) or of wave bit k - 11 (k < 14).
+struct PhaseBits {
+    unsigned bits[4];
+    int op_begin, n_ops;
+    __host__ __device__ int byte(int k) const { return (int)((bits[k >> 2] >> ((k & 3) * 8)) & 0xffu); }
+    __host__ __device__ int rb(int k) const { return byte(k); }
+    __host__ __device__ int lb(int k) const { return byte(5 + k); }
+    __host__ __device__ int wb(int k) const { return byte(11 + k); }
+    void set(int k, int v) {
+        const int sh = (k & 3) * 8;
+        bits[k >> 2] = (bits[k >> 2] & ~(0xffu << sh)) | ((unsigned)v << sh);
+    }
 };
+typedef PhaseBits RegPhase;
 
 template <class R>
 struct RegProgram {
     int n_ops, n_phases, low;
-    signed char pos[16];
+    alignas(4) signed char pos[kTilePosMax];
     RegPhase phases[kMaxPhases];
     TileOp<R> ops[TileCfg<R>::MaxOps];   // pair gates: target = register position
 };
@@ -151,6 +162,15 @@ __device__ __forceinline__ void static_for(F &&f) {
         f(std::integral_constant<int, B>());
         static_for<B + 1, E>(f);
     }
+}
+
+// Ends a register-pair update on register bit P. The markers differ per P, so the optimiser
+// cannot merge the updates of different bits (dispatched on a runtime bit) into one block
+// that picks the register elements through pointers, which would move those elements to
+// private (scratch) memory.
+template <int P>
+__device__ __forceinline__ void pair_marker() {
+    asm volatile("; register pair %0" ::"n"(P));
 }
 
 // Pair update on register bit P. F = 0: general matrix; 1: swap; 2: real matrix.
@@ -176,6 +196,7 @@ __device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const unsigned (&lidx)[A
             }
         }
     });
+    pair_marker<P * 4 + F>();
 }
 
 template <class R, int A, int F>
@@ -216,14 +237,14 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
         const RegPhase P = prog.phases[ph];
         unsigned off = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb[k];
+        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb(k);
 #pragma unroll
-        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb[k];
+        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
         static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
             unsigned l = off;
 #pragma unroll
-            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb[k];
+            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb(k);
             lidx[j] = l;
             v[j] = t[l];
         });
@@ -290,7 +311,9 @@ std::vector<PhaseLayout> plan_phase_layouts(const std::vector<int> &pair, int m,
             P.regs.push_back(pair[k]);
         }
         P.end = k;
-        for (int b = 0; (int)P.regs.size() < rb_count && b < m; ++b)
+        // pad with the highest free bits, so that the lane bits are the lowest ones: a
+        // wavefront then reads and writes LDS at consecutive addresses (no bank conflicts)
+        for (int b = m - 1; (int)P.regs.size() < rb_count && b >= 0; --b)
             if (std::find(P.regs.begin(), P.regs.end(), b) == P.regs.end()) P.regs.push_back(b);
         std::sort(P.regs.begin(), P.regs.end());
         for (int b = 0; b < m; ++b)
@@ -322,11 +345,11 @@ bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
     for (int i = 0; i < prog.n_phases; ++i) {
         RegPhase &P = prog.phases[i];
         std::memset(&P, 0, sizeof(P));
-        for (int b = 0; b < rb_count; ++b) P.rb[b] = (signed char)L[i].regs[b];
-        for (int b = 0; b < 6; ++b) P.lb[b] = (signed char)L[i].lanes[b];
-        for (int b = 0; b < wb_count; ++b) P.wb[b] = (signed char)L[i].waves[b];
-        P.op_begin = (unsigned char)L[i].begin;
-        P.n_ops = (unsigned char)(L[i].end - L[i].begin);
+        for (int b = 0; b < rb_count; ++b) P.set(b, L[i].regs[b]);
+        for (int b = 0; b < 6; ++b) P.set(5 + b, L[i].lanes[b]);
+        for (int b = 0; b < wb_count; ++b) P.set(11 + b, L[i].waves[b]);
+        P.op_begin = L[i].begin;
+        P.n_ops = L[i].end - L[i].begin;
         for (int k = L[i].begin; k < L[i].end; ++k)
             if (is_pair_kind(prog.ops[k].kind)) prog.ops[k].target = (short)reg_position(L[i], prog.ops[k].target);
     }
@@ -449,8 +472,8 @@ struct SweepOp {
     Mat2<R> gen;        // generator of a rotation (entries 0, +-1, +-i)
     u64 gctrl;
     unsigned lctrl;
-    short target;       // local bit (pair, diag-local) or state bit (diag-outer)
-    short kind;
+    int target;         // local bit (pair, diag-local) or state bit (diag-outer)
+    int kind;           // (dwords: a uniform dword field is one scalar load)
     int slot;           // gradient slot within the pass, -1 for fixed gates
 };
 
@@ -458,7 +481,7 @@ struct SweepPass {
     int m, low;
     int op_begin, n_ops;
     int phase_begin, n_phases, n_slots;
-    signed char pos[16];
+    alignas(4) signed char pos[kTilePosMax];
 };
 
 // One pass of the adjoint backward sweep over a phi tile and a lambda tile: for each op in
@@ -477,8 +500,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
     const int m = pass.m;
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, m);
-    tile_to_lds<R, kTileThreads, 4>(tp, phi, base, pass.pos, m, pass.low);
-    tile_to_lds<R, kTileThreads, 4>(tl, lam, base, pass.pos, m, pass.low);
+    load_tile_lds<R, kTileThreads, MB>(tp, phi, base, pass.pos, m, pass.low);
+    load_tile_lds<R, kTileThreads, MB>(tl, lam, base, pass.pos, m, pass.low);
     __syncthreads();
     const int n_amps = 1 << m;
     for (int k = 0; k < pass.n_ops; ++k) {
@@ -527,8 +550,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
         }
         __syncthreads();
     }
-    lds_to_tile<R, kTileThreads, 4>(phi, tp, base, pass.pos, m, pass.low);
-    lds_to_tile<R, kTileThreads, 4>(lam, tl, base, pass.pos, m, pass.low);
+    store_tile_lds<R, kTileThreads, MB>(phi, tp, base, pass.pos, m, pass.low);
+    store_tile_lds<R, kTileThreads, MB>(lam, tl, base, pass.pos, m, pass.low);
 }
 
 // Register-resident sweep pass (m == SweepBits): phi and lambda tiles in registers, one LDS
@@ -536,10 +559,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_sweep(Cx<R> *__restrict__
 // over the workgroup at the end of the pass.
 constexpr int kMaxSweepSlots = 96;
 
-struct SweepPhase {
-    signed char rb[5], lb[6], wb[3];
-    short op_begin, n_ops;   // relative to the pass
-};
+typedef PhaseBits SweepPhase;   // op_begin relative to the pass
 
 template <class R, int A, int P>
 __device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const unsigned (&lidx)[A],
@@ -560,6 +580,7 @@ __device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], c
             vl[j + st] = cadd(cmul(ud.m10, l0), cmul(ud.m11, l1));
         }
     }
+    pair_marker<P>();
 }
 
 template <class R, int A>
@@ -604,14 +625,14 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
         const SweepPhase P = phases[pass.phase_begin + ph];
         unsigned off = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb[k];
+        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb(k);
 #pragma unroll
-        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb[k];
+        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
 #pragma unroll
         for (int j = 0; j < A; ++j) {
             unsigned l = off;
 #pragma unroll
-            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb[k];
+            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb(k);
             lidx[j] = l;
             vp[j] = tp[l];
             vl[j] = tl[l];
@@ -704,7 +725,7 @@ struct PauliPass {
     int g_begin, n_groups;
     int accumulate;     // add to lambda (later passes) instead of writing it
     int dot;            // last pass: partial[tile] = Re<psi|lambda> over the tile
-    signed char pos[16];
+    alignas(4) signed char pos[kTilePosMax];
 };
 
 // lambda (+)= sum over this pass's groups g of sum_{k in g} c_k (-1)^popcount(x & zm_k) psi[x ^ xm_g]
@@ -722,7 +743,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pauli(const Cx<R> *__rest
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, m);
     const int n_units = 1 << (m - lw);
-    tile_to_lds<R, kTileThreads>(tp, psi, base, pass.pos, m, pass.low);
+    load_tile_lds<R, kTileThreads, MB>(tp, psi, base, pass.pos, m, pass.low);
     __syncthreads();
     double dot = 0.;
     const u64 dt = base | deposit((unsigned)threadIdx.x << lw, pass.pos, m, pass.low);
@@ -906,11 +927,11 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
             for (const PhaseLayout &P : L) {
                 SweepPhase h;
                 std::memset(&h, 0, sizeof(h));
-                for (int b = 0; b < (int)P.regs.size(); ++b) h.rb[b] = (signed char)P.regs[b];
-                for (int b = 0; b < 6; ++b) h.lb[b] = (signed char)P.lanes[b];
-                for (int b = 0; b < (int)P.waves.size(); ++b) h.wb[b] = (signed char)P.waves[b];
-                h.op_begin = (short)P.begin;
-                h.n_ops = (short)(P.end - P.begin);
+                for (int b = 0; b < (int)P.regs.size(); ++b) h.set(b, P.regs[b]);
+                for (int b = 0; b < 6; ++b) h.set(5 + b, P.lanes[b]);
+                for (int b = 0; b < (int)P.waves.size(); ++b) h.set(11 + b, P.waves[b]);
+                h.op_begin = P.begin;
+                h.n_ops = P.end - P.begin;
                 for (int k = P.begin; k < P.end; ++k)
                     if (hops[pl.begin + k].kind == kOpPair)
                         hops[pl.begin + k].target = (short)reg_position(P, hops[pl.begin + k].target);
