@@ -137,12 +137,43 @@ struct PhaseBits {
 };
 typedef PhaseBits RegPhase;
 
+// Local-index bits of register index j in phase P (uniform: a scalar value).
+template <int RB>
+__device__ __forceinline__ unsigned reg_bits(const PhaseBits &P, int j) {
+    unsigned l = 0;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb(k);
+    return l;
+}
+
+// Register-pass op kinds. Pair updates on a register position (general, swap, real); a
+// diagonal whose target is a register position (every amplitude, or only the |1> half when
+// m00 == 1); a diagonal whose target is a lane/wave bit or a bit outside the tile, which is
+// one factor per thread.
+enum : int {
+    kRegPair = 0, kRegPairSwap = 1, kRegPairReal = 2,
+    kRegDiag = 3, kRegDiagPhase = 4, kRegDiagThread = 5, kRegDiagOuter = 6
+};
+
+// A gate in a register phase. Its controls are split by where their bits live: outside the
+// tile (one test per tile), on lane/wave bits (one test per thread) and on register bits
+// (register positions: tests on the compile-time register indices, made by the scalar unit).
+template <class R>
+struct RegOp {
+    Mat2<R> m;
+    u64 gctrl;         // controls outside the tile (state bits)
+    unsigned cthr;     // controls on lane/wave bits (local bits)
+    int creg;          // controls on register bits (mask of register positions)
+    int target;        // register position (pairs, kRegDiag*), local bit (kRegDiagThread), state bit (kRegDiagOuter)
+    int kind;
+};
+
 template <class R>
 struct RegProgram {
     int n_ops, n_phases, low;
     alignas(4) signed char pos[kTilePosMax];
     RegPhase phases[kMaxPhases];
-    TileOp<R> ops[TileCfg<R>::MaxOps];   // pair gates: target = register position
+    RegOp<R> ops[TileCfg<R>::MaxOps];
 };
 
 template <int A>
@@ -173,50 +204,160 @@ __device__ __forceinline__ void pair_marker() {
     asm volatile("; register pair %0" ::"n"(P));
 }
 
-// Pair update on register bit P. F = 0: general matrix; 1: swap; 2: real matrix.
+// Im(conj(a) b)
+template <class R>
+__device__ __forceinline__ R im_conj_mul(const Cx<R> &a, const Cx<R> &b) {
+    return a.re * b.im - a.im * b.re;
+}
+
+// c ? a : b, component by component: a select of whole structs can be lowered to a select
+// of their addresses, which would keep register arrays out of registers.
+template <class R>
+__device__ __forceinline__ Cx<R> csel(bool c, const Cx<R> &a, const Cx<R> &b) {
+    return cx<R>(c ? a.re : b.re, c ? a.im : b.im);
+}
+
+// (y0, y1) = M (x0, x1) for a matrix of form F: 0 general, 1 swap, 2 real, 3 real diagonal
+// and imaginary off-diagonal (RX-like: m00, m11 real; m01, m10 imaginary).
+template <class R, int F>
+__device__ __forceinline__ void apply_form(const Mat2<R> &mm, const Cx<R> &x0, const Cx<R> &x1, Cx<R> &y0, Cx<R> &y1) {
+    if constexpr (F == 1) {
+        y0 = x1;
+        y1 = x0;
+    } else if constexpr (F == 2) {
+        const R a = mm.m00.re, b = mm.m01.re, c = mm.m10.re, d = mm.m11.re;
+        y0 = cx<R>(a * x0.re + b * x1.re, a * x0.im + b * x1.im);
+        y1 = cx<R>(c * x0.re + d * x1.re, c * x0.im + d * x1.im);
+    } else if constexpr (F == 3) {
+        const R a = mm.m00.re, b = mm.m01.im, c = mm.m10.im, d = mm.m11.re;
+        y0 = cx<R>(a * x0.re - b * x1.im, a * x0.im + b * x1.re);
+        y1 = cx<R>(d * x1.re - c * x0.im, d * x1.im + c * x0.re);
+    } else {
+        y0 = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
+        y1 = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
+    }
+}
+
+// Pair update on register bit P (form F, see apply_form) of the register indices j whose
+// register-position controls creg are set.
 template <class R, int A, int P, int F>
-__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc) {
+__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const Mat2<R> &mm, int creg) {
     static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
         if constexpr (((j >> P) & 1) == 0) {
             constexpr int j1 = j + (1 << P);
-            if ((lidx[j] & lc) == lc) {
+            if ((j & creg) == creg) {
                 const Cx<R> x0 = v[j], x1 = v[j1];
-                if constexpr (F == 1) {
-                    v[j] = x1;
-                    v[j1] = x0;
-                } else if constexpr (F == 2) {
-                    const R a = mm.m00.re, b = mm.m01.re, c = mm.m10.re, d = mm.m11.re;
-                    v[j] = cx<R>(a * x0.re + b * x1.re, a * x0.im + b * x1.im);
-                    v[j1] = cx<R>(c * x0.re + d * x1.re, c * x0.im + d * x1.im);
-                } else {
-                    v[j] = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
-                    v[j1] = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
-                }
+                apply_form<R, F>(mm, x0, x1, v[j], v[j1]);
             }
+            __builtin_amdgcn_sched_barrier(0);   // one pair at a time: bounded register use
         }
     });
     pair_marker<P * 4 + F>();
 }
 
+// Pair update with controls: only register indices with (j & creg) == creg (scalar tests)
+// in threads with ok set change; written as selects, so no masked region needs copies of
+// the registers.
+template <class R, int A, int P, int F>
+__device__ __forceinline__ void reg_pair_sel(Cx<R> (&v)[A], const Mat2<R> &mm, int creg, bool ok) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (((j >> P) & 1) == 0) {
+            constexpr int j1 = j + (1 << P);
+            const bool c = ok && (j & creg) == creg;
+            const Cx<R> x0 = v[j], x1 = v[j1];
+            Cx<R> y0, y1;
+            apply_form<R, F>(mm, x0, x1, y0, y1);
+            v[j] = csel(c, y0, x0);
+            v[j1] = csel(c, y1, x1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    });
+    pair_marker<32 + P * 4 + F>();
+}
+
+// Dispatch on the register position p (< RB, guaranteed by the planner) as one switch with
+// an unreachable default: every path ends at one join. A chain of one-armed ifs would give
+// each not-taken arm a copy of the whole register array.
 template <class R, int A, int F>
-__device__ __forceinline__ void reg_pair_at(Cx<R> (&v)[A], const unsigned (&lidx)[A], const Mat2<R> &mm, unsigned lc,
-                                            int p) {
+__device__ __forceinline__ void reg_pair_sel_at(Cx<R> (&v)[A], const Mat2<R> &mm, int creg, bool ok, int p) {
+    constexpr int RB = Log2<A>::v;
+    switch (p) {
+        case 0:
+            if constexpr (RB > 0) {
+                reg_pair_sel<R, A, 0, F>(v, mm, creg, ok);
+                break;
+            }
+            [[fallthrough]];
+        case 1:
+            if constexpr (RB > 1) {
+                reg_pair_sel<R, A, 1, F>(v, mm, creg, ok);
+                break;
+            }
+            [[fallthrough]];
+        case 2:
+            if constexpr (RB > 2) {
+                reg_pair_sel<R, A, 2, F>(v, mm, creg, ok);
+                break;
+            }
+            [[fallthrough]];
+        case 3:
+            if constexpr (RB > 3) {
+                reg_pair_sel<R, A, 3, F>(v, mm, creg, ok);
+                break;
+            }
+            [[fallthrough]];
+        default: __builtin_unreachable();
+    }
+}
+
+template <class R, int A, int F>
+__device__ __forceinline__ void reg_pair_if(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
     constexpr int RB = Log2<A>::v;
     if constexpr (RB > 0) {
-        if (p == 0) reg_pair<R, A, 0, F>(v, lidx, mm, lc);
+        if (p == 0) reg_pair<R, A, 0, F>(v, mm, 0);
     }
     if constexpr (RB > 1) {
-        if (p == 1) reg_pair<R, A, 1, F>(v, lidx, mm, lc);
+        if (p == 1) reg_pair<R, A, 1, F>(v, mm, 0);
     }
     if constexpr (RB > 2) {
-        if (p == 2) reg_pair<R, A, 2, F>(v, lidx, mm, lc);
+        if (p == 2) reg_pair<R, A, 2, F>(v, mm, 0);
     }
     if constexpr (RB > 3) {
-        if (p == 3) reg_pair<R, A, 3, F>(v, lidx, mm, lc);
+        if (p == 3) reg_pair<R, A, 3, F>(v, mm, 0);
     }
-    if constexpr (RB > 4) {
-        if (p == 4) reg_pair<R, A, 4, F>(v, lidx, mm, lc);
+}
+
+template <class R, int A, int F>
+__device__ __forceinline__ void reg_pair_at(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
+    constexpr int RB = Log2<A>::v;
+    switch (p) {
+        case 0:
+            if constexpr (RB > 0) {
+                reg_pair<R, A, 0, F>(v, mm, 0);
+                break;
+            }
+            [[fallthrough]];
+        case 1:
+            if constexpr (RB > 1) {
+                reg_pair<R, A, 1, F>(v, mm, 0);
+                break;
+            }
+            [[fallthrough]];
+        case 2:
+            if constexpr (RB > 2) {
+                reg_pair<R, A, 2, F>(v, mm, 0);
+                break;
+            }
+            [[fallthrough]];
+        case 3:
+            if constexpr (RB > 3) {
+                reg_pair<R, A, 3, F>(v, mm, 0);
+                break;
+            }
+            [[fallthrough]];
+        default: __builtin_unreachable();
     }
 }
 
@@ -232,7 +373,6 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
     __syncthreads();
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Cx<R> v[A];
-    unsigned lidx[A];
     for (int ph = 0; ph < prog.n_phases; ++ph) {
         const RegPhase P = prog.phases[ph];
         unsigned off = 0;
@@ -242,47 +382,78 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
         for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
         static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            unsigned l = off;
-#pragma unroll
-            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb(k);
-            lidx[j] = l;
-            v[j] = t[l];
+            v[j] = t[off | reg_bits<RB>(P, j)];
         });
+        // Diagonal gates with no register bit among their bits multiply all of a thread's
+        // amplitudes by one factor. They are collected in tf and applied once at the end of
+        // the phase: tf commutes with every other gate of the phase, which mixes or scales
+        // amplitudes of one thread only.
+        Cx<R> tf = cx<R>(R(1), R(0));
+        bool pending = false;
         for (int k = 0; k < P.n_ops; ++k) {
             // plain locals: the lambdas below must not capture the kernel argument by address
-            const u64 gctrl = prog.ops[P.op_begin + k].gctrl;
+            const int oi = P.op_begin + k;
+            const u64 gctrl = prog.ops[oi].gctrl;
             if ((base & gctrl) != gctrl) continue;   // same for the whole workgroup
-            const Mat2<R> mm = prog.ops[P.op_begin + k].m;
-            const unsigned lc = prog.ops[P.op_begin + k].lctrl;
-            const int tgt = prog.ops[P.op_begin + k].target;
-            const int kind = prog.ops[P.op_begin + k].kind;
-            if (kind == kOpPair) reg_pair_at<R, A, 0>(v, lidx, mm, lc, tgt);
-            if (kind == kOpPairSwap) reg_pair_at<R, A, 1>(v, lidx, mm, lc, tgt);
-            if (kind == kOpPairReal) reg_pair_at<R, A, 2>(v, lidx, mm, lc, tgt);
-            if (kind == kOpDiagLocal) {
-                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    if ((lidx[j] & lc) == lc) v[j] = cmul(((lidx[j] >> tgt) & 1u) ? mm.m11 : mm.m00, v[j]);
-                });
+            const Mat2<R> mm = prog.ops[oi].m;
+            const unsigned cthr = prog.ops[oi].cthr;
+            const int creg = prog.ops[oi].creg, tgt = prog.ops[oi].target, kind = prog.ops[oi].kind;
+            // no controls on lane, wave or register bits: straight-line updates, no tests
+            const bool plain = cthr == 0 && creg == 0;
+            const bool ok = (off & cthr) == cthr;
+            if (kind <= kRegPairReal) {
+                // one-armed ifs here (not one switch): they cost register copies per gate but
+                // keep the kernel near 110 VGPRs, i.e. 4 workgroups per CU, which measured faster
+                if (plain) {
+                    if (kind == kRegPair) reg_pair_if<R, A, 0>(v, mm, tgt);
+                    if (kind == kRegPairSwap) reg_pair_if<R, A, 1>(v, mm, tgt);
+                    if (kind == kRegPairReal) reg_pair_if<R, A, 2>(v, mm, tgt);
+                } else {
+                    if (kind == kRegPair) reg_pair_sel_at<R, A, 0>(v, mm, creg, ok, tgt);
+                    if (kind == kRegPairSwap) reg_pair_sel_at<R, A, 1>(v, mm, creg, ok, tgt);
+                    if (kind == kRegPairReal) reg_pair_sel_at<R, A, 2>(v, mm, creg, ok, tgt);
+                }
+            } else if (kind <= kRegDiagPhase) {
+                // target on register position tgt: the factor of index j is a scalar choice
+                if (plain) {
+                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        v[j] = cmul(csel(((j >> tgt) & 1) != 0, mm.m11, mm.m00), v[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    });
+                } else {
+                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const Cx<R> y = cmul(csel(((j >> tgt) & 1) != 0, mm.m11, mm.m00), v[j]);
+                        v[j] = csel(ok && (j & creg) == creg, y, v[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    });
+                }
+            } else {
+                const u64 src = kind == kRegDiagThread ? (u64)off : base;
+                Cx<R> f = csel(((src >> tgt) & 1) != 0, mm.m11, mm.m00);
+                f = csel(cthr == 0 || ok, f, cx<R>(R(1), R(0)));
+                if (creg == 0) {
+                    tf = cmul(tf, f);
+                    pending = true;
+                } else {
+                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        if ((j & creg) == creg) v[j] = cmul(f, v[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    });
+                }
             }
-            if (kind == kOpDiagPhase) {
-                const unsigned need = lc | (1u << tgt);
-                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    if ((lidx[j] & need) == need) v[j] = cmul(mm.m11, v[j]);
-                });
-            }
-            if (kind == kOpDiagOuter) {
-                const Cx<R> f = ((base >> tgt) & 1) ? mm.m11 : mm.m00;
-                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    if ((lidx[j] & lc) == lc) v[j] = cmul(f, v[j]);
-                });
-            }
+        }
+        if (pending) {
+            static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                v[j] = cmul(tf, v[j]);
+            });
         }
         static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            t[lidx[j]] = v[j];
+            t[off | reg_bits<RB>(P, j)] = v[j];
         });
         __syncthreads();
     }
@@ -335,10 +506,52 @@ int reg_position(const PhaseLayout &P, int bit) {
 
 // Phase layouts of a register program: ops[k].target holds local bits on entry; pair gates
 // get their register position. Returns false if the pass needs more phases than it holds.
+// Register-pass encoding of a gate in phase P: kind (tile-pass kind, local bits: kOpPair*,
+// kOpDiag*), target (local bit, or state bit for kOpDiagOuter) and local controls lctrl.
+struct RegSplit {
+    int kind, target, creg;
+    unsigned cthr;
+};
+
+RegSplit split_for_phase(const PhaseLayout &P, int kind, int target, unsigned lctrl) {
+    RegSplit r;
+    r.creg = 0;
+    r.cthr = 0;
+    const int rb = (int)P.regs.size();
+    for (int b = 0; b < 32; ++b) {
+        if (((lctrl >> b) & 1u) == 0) continue;
+        const int p = reg_position(P, b);
+        if (p < rb)
+            r.creg |= 1 << p;
+        else
+            r.cthr |= 1u << b;
+    }
+    if (kind == kOpPair || kind == kOpPairSwap || kind == kOpPairReal) {
+        r.kind = kind == kOpPair ? kRegPair : kind == kOpPairSwap ? kRegPairSwap : kRegPairReal;
+        r.target = reg_position(P, target);
+        QG_REQUIRE(r.target < rb, "register phase: pair target is not a register bit");
+    } else if (kind == kOpDiagOuter) {
+        r.kind = kRegDiagOuter;
+        r.target = target;
+    } else {
+        const int p = reg_position(P, target);
+        if (p < rb) {
+            r.kind = kind == kOpDiagPhase ? kRegDiagPhase : kRegDiag;
+            r.target = p;
+        } else {
+            r.kind = kRegDiagThread;
+            r.target = target;
+        }
+    }
+    return r;
+}
+
+// Phase layouts of a register program built from tile-pass ops (local bits): false if the
+// pass needs more phases than a program holds.
 template <class R>
-bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
+bool plan_phases(RegProgram<R> &prog, const TileOp<R> *ops, int m, int rb_count, int wb_count) {
     std::vector<int> pair(prog.n_ops);
-    for (int k = 0; k < prog.n_ops; ++k) pair[k] = is_pair_kind(prog.ops[k].kind) ? prog.ops[k].target : -1;
+    for (int k = 0; k < prog.n_ops; ++k) pair[k] = is_pair_kind(ops[k].kind) ? ops[k].target : -1;
     const std::vector<PhaseLayout> L = plan_phase_layouts(pair, m, rb_count, wb_count);
     if ((int)L.size() > kMaxPhases) return false;
     prog.n_phases = (int)L.size();
@@ -350,8 +563,16 @@ bool plan_phases(RegProgram<R> &prog, int m, int rb_count, int wb_count) {
         for (int b = 0; b < wb_count; ++b) P.set(11 + b, L[i].waves[b]);
         P.op_begin = L[i].begin;
         P.n_ops = L[i].end - L[i].begin;
-        for (int k = L[i].begin; k < L[i].end; ++k)
-            if (is_pair_kind(prog.ops[k].kind)) prog.ops[k].target = (short)reg_position(L[i], prog.ops[k].target);
+        for (int k = L[i].begin; k < L[i].end; ++k) {
+            const RegSplit r = split_for_phase(L[i], ops[k].kind, ops[k].target, ops[k].lctrl);
+            RegOp<R> &op = prog.ops[k];
+            op.m = ops[k].m;
+            op.gctrl = ops[k].gctrl;
+            op.cthr = r.cthr;
+            op.creg = r.creg;
+            op.target = r.target;
+            op.kind = r.kind;
+        }
     }
     return true;
 }
@@ -432,9 +653,10 @@ void launch_pass(Cx<R> *a, int n_lanes, const PassPlan &plan, int low, int max_b
         rp.n_ops = prog.n_ops;
         rp.low = prog.low;
         std::memcpy(rp.pos, prog.pos, sizeof(rp.pos));
+        TileOp<R> tops[C::MaxOps];
         for (int k = 0; k < prog.n_ops; ++k) {
-            rp.ops[k] = prog.ops[k];
-            TileOp<R> &op = rp.ops[k];
+            tops[k] = prog.ops[k];
+            TileOp<R> &op = tops[k];
             const Mat2<R> &mm = op.m;
             const R zero = R(0), one = R(1);
             if (op.kind == kOpPair) {
@@ -447,7 +669,7 @@ void launch_pass(Cx<R> *a, int n_lanes, const PassPlan &plan, int low, int max_b
             }
         }
         constexpr int A = (1 << C::MaxBits) / kTileThreads;
-        if (plan_phases<R>(rp, prog.m, Log2<A>::v, Log2<kTileThreads / 64>::v)) {
+        if (plan_phases<R>(rp, tops, prog.m, Log2<A>::v, Log2<kTileThreads / 64>::v)) {
             for (u64 first = 0; first < n_tiles; first += piece) {
                 const u64 count = std::min(piece, n_tiles - first);
                 hipLaunchKernelGGL((k_tile_reg<R, C::MaxBits, kTileThreads>), (unsigned)count, kTileThreads, 0, s, a,
@@ -475,6 +697,10 @@ struct SweepOp {
     int target;         // local bit (pair, diag-local) or state bit (diag-outer)
     int kind;           // (dwords: a uniform dword field is one scalar load)
     int slot;           // gradient slot within the pass, -1 for fixed gates
+    int form;           // register passes: 4 * matrix form + generator (sweep_form)
+    int rkind, rtarget; // register passes: kind and target in the phase (RegOp)
+    int creg;           // register passes: controls on register bits (register positions)
+    unsigned cthr;      // register passes: controls on lane/wave bits (local bits)
 };
 
 struct SweepPass {
@@ -561,44 +787,119 @@ constexpr int kMaxSweepSlots = 96;
 
 typedef PhaseBits SweepPhase;   // op_begin relative to the pass
 
-template <class R, int A, int P>
-__device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const unsigned (&lidx)[A],
-                                               const Mat2<R> &ud, const Mat2<R> &gen, unsigned lc, double &s) {
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-        if ((j >> P) & 1) continue;
-        constexpr int st = 1 << P;
-        if ((lidx[j] & lc) == lc) {
-            const Cx<R> p0 = vp[j], p1 = vp[j + st], l0 = vl[j], l1 = vl[j + st];
-            const Cx<R> q0 = cadd(cmul(gen.m00, p0), cmul(gen.m01, p1));
-            const Cx<R> q1 = cadd(cmul(gen.m10, p0), cmul(gen.m11, p1));
-            s += (double)l0.re * (double)q0.im - (double)l0.im * (double)q0.re + (double)l1.re * (double)q1.im -
-                 (double)l1.im * (double)q1.re;
-            vp[j] = cadd(cmul(ud.m00, p0), cmul(ud.m01, p1));
-            vp[j + st] = cadd(cmul(ud.m10, p0), cmul(ud.m11, p1));
-            vl[j] = cadd(cmul(ud.m00, l0), cmul(ud.m01, l1));
-            vl[j + st] = cadd(cmul(ud.m10, l0), cmul(ud.m11, l1));
+// Register sweep pair update on register bit P: the gradient term Im<lambda|G|phi> over the
+// pair for generator G (0 none, 1 X, 2 Y, 3 Z), accumulated in `acc` (state precision; the
+// caller widens each thread's sum to FP64), then U^dagger of form F on phi and lambda.
+// SEL: controls present; a pair changes (and contributes) only if ok && (j & creg) == creg,
+// written as selects so that no masked region needs copies of the registers.
+template <class R, int A, int P, int F, int G, bool SEL>
+__device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const Mat2<R> &ud, int creg, bool ok,
+                                               R &acc) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (((j >> P) & 1) == 0) {
+            constexpr int j1 = j + (1 << P);
+            const Cx<R> p0 = vp[j], p1 = vp[j1], l0 = vl[j], l1 = vl[j1];
+            R t = R(0);
+            if constexpr (G == 1) t = im_conj_mul(l0, p1) + im_conj_mul(l1, p0);
+            if constexpr (G == 2) t = (l1.re * p0.re + l1.im * p0.im) - (l0.re * p1.re + l0.im * p1.im);
+            if constexpr (G == 3) t = im_conj_mul(l0, p0) - im_conj_mul(l1, p1);
+            Cx<R> q0, q1, m0, m1;
+            apply_form<R, F>(ud, p0, p1, q0, q1);
+            apply_form<R, F>(ud, l0, l1, m0, m1);
+            if constexpr (SEL) {
+                const bool c = ok && (j & creg) == creg;
+                if constexpr (G != 0) acc += c ? t : R(0);
+                vp[j] = csel(c, q0, p0);
+                vp[j1] = csel(c, q1, p1);
+                vl[j] = csel(c, m0, l0);
+                vl[j1] = csel(c, m1, l1);
+            } else {
+                if constexpr (G != 0) acc += t;
+                vp[j] = q0;
+                vp[j1] = q1;
+                vl[j] = m0;
+                vl[j1] = m1;
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
-    }
-    pair_marker<P>();
+    });
+    pair_marker<64 + ((P * 4 + F) * 4 + G) * 2 + (SEL ? 1 : 0)>();
 }
 
-template <class R, int A>
-__device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const unsigned (&lidx)[A],
-                                                  const Mat2<R> &ud, const Mat2<R> &gen, unsigned lc, int p,
-                                                  double &s) {
+template <class R, int A, int F, int G, bool SEL>
+__device__ __forceinline__ void reg_sweep_pair_pos(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const Mat2<R> &ud, int creg, bool ok,
+                                                   int p, R &acc) {
     constexpr int RB = Log2<A>::v;
-    if constexpr (RB > 0) {
-        if (p == 0) reg_sweep_pair<R, A, 0>(vp, vl, lidx, ud, gen, lc, s);
+    if constexpr (sizeof(R) == 8) {
+        // complex128: one-armed ifs keep the kernel's register count (and occupancy) lower
+        if constexpr (RB > 0) {
+            if (p == 0) reg_sweep_pair<R, A, 0, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+        }
+        if constexpr (RB > 1) {
+            if (p == 1) reg_sweep_pair<R, A, 1, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+        }
+        if constexpr (RB > 2) {
+            if (p == 2) reg_sweep_pair<R, A, 2, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+        }
+        if constexpr (RB > 3) {
+            if (p == 3) reg_sweep_pair<R, A, 3, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+        }
+        return;
     }
-    if constexpr (RB > 1) {
-        if (p == 1) reg_sweep_pair<R, A, 1>(vp, vl, lidx, ud, gen, lc, s);
+    switch (p) {
+        case 0:
+            if constexpr (RB > 0) {
+                reg_sweep_pair<R, A, 0, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+                break;
+            }
+            [[fallthrough]];
+        case 1:
+            if constexpr (RB > 1) {
+                reg_sweep_pair<R, A, 1, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+                break;
+            }
+            [[fallthrough]];
+        case 2:
+            if constexpr (RB > 2) {
+                reg_sweep_pair<R, A, 2, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+                break;
+            }
+            [[fallthrough]];
+        case 3:
+            if constexpr (RB > 3) {
+                reg_sweep_pair<R, A, 3, F, G, SEL>(vp, vl, ud, creg, ok, acc);
+                break;
+            }
+            [[fallthrough]];
+        default: __builtin_unreachable();
     }
-    if constexpr (RB > 2) {
-        if (p == 2) reg_sweep_pair<R, A, 2>(vp, vl, lidx, ud, gen, lc, s);
+}
+
+// Pair op of form code 4 F + G (see sweep_form): fixed gates in the general, swap and real
+// forms, RY (real, Y) and RX (form 3, X); any other combination takes the general matrix.
+template <class R, int A>
+__device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const Mat2<R> &ud, int creg, bool ok,
+                                                  bool plain, int p, int form, R &acc) {
+    if (plain) {
+        switch (form) {
+            case 4 * 2 + 2: reg_sweep_pair_pos<R, A, 2, 2, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 4 * 3 + 1: reg_sweep_pair_pos<R, A, 3, 1, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 4 * 1 + 0: reg_sweep_pair_pos<R, A, 1, 0, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 4 * 2 + 0: reg_sweep_pair_pos<R, A, 2, 0, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 1: reg_sweep_pair_pos<R, A, 0, 1, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 2: reg_sweep_pair_pos<R, A, 0, 2, false>(vp, vl, ud, 0, true, p, acc); break;
+            case 3: reg_sweep_pair_pos<R, A, 0, 3, false>(vp, vl, ud, 0, true, p, acc); break;
+            default: reg_sweep_pair_pos<R, A, 0, 0, false>(vp, vl, ud, 0, true, p, acc); break;
+        }
+        return;
     }
-    if constexpr (RB > 3) {
-        if (p == 3) reg_sweep_pair<R, A, 3>(vp, vl, lidx, ud, gen, lc, s);
+    // controlled gates: general matrix, any generator
+    switch (form & 3) {
+        case 1: reg_sweep_pair_pos<R, A, 0, 1, true>(vp, vl, ud, creg, ok, p, acc); break;
+        case 2: reg_sweep_pair_pos<R, A, 0, 2, true>(vp, vl, ud, creg, ok, p, acc); break;
+        case 3: reg_sweep_pair_pos<R, A, 0, 3, true>(vp, vl, ud, creg, ok, p, acc); break;
+        default: reg_sweep_pair_pos<R, A, 0, 0, true>(vp, vl, ud, creg, ok, p, acc); break;
     }
 }
 
@@ -620,7 +921,6 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
     __syncthreads();
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Cx<R> vp[A], vl[A];
-    unsigned lidx[A];
     for (int ph = 0; ph < pass.n_phases; ++ph) {
         const SweepPhase P = phases[pass.phase_begin + ph];
         unsigned off = 0;
@@ -628,47 +928,112 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
         for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb(k);
 #pragma unroll
         for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
-#pragma unroll
-        for (int j = 0; j < A; ++j) {
-            unsigned l = off;
-#pragma unroll
-            for (int k = 0; k < RB; ++k) l |= (((unsigned)j >> k) & 1u) << P.rb(k);
-            lidx[j] = l;
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const unsigned l = off | reg_bits<RB>(P, j);
             vp[j] = tp[l];
             vl[j] = tl[l];
-        }
+        });
+        // As in the forward register pass, diagonal gates on no register bit are one factor per
+        // thread (tf), applied to phi and lambda at the end of the phase. Diagonal gates
+        // leave w_j = Im(conj(lambda_j) phi_j) unchanged (|factor| = 1), so the generator-Z
+        // terms of a run of diagonal gates all use the same w: an RZ on a lane, wave or outer
+        // bit contributes +-U with U = sum_j w_j, computed once per run (have_u).
+        Cx<R> tf = cx<R>(R(1), R(0));
+        bool pending = false, have_u = false;
+        R u_sum = R(0);
         for (int k = 0; k < P.n_ops; ++k) {
             const SweepOp<R> &op = ops[pass.op_begin + P.op_begin + k];
             const bool on = (base & op.gctrl) == op.gctrl;
-            const unsigned lc = op.lctrl;
-            double s = 0.;
-            if (on && op.kind == kOpPair) reg_sweep_pair_at<R, A>(vp, vl, lidx, op.udag, op.gen, lc, op.target, s);
-            if (on && op.kind != kOpPair) {
-                const bool outer = op.kind == kOpDiagOuter;
-                const bool obit = ((base >> op.target) & 1) != 0;
-#pragma unroll
-                for (int j = 0; j < A; ++j) {
-                    if ((lidx[j] & lc) != lc) continue;
-                    const bool one = outer ? obit : ((lidx[j] >> op.target) & 1u) != 0;
-                    const Cx<R> x = vp[j], y = vl[j];
-                    const Cx<R> q = cmul(one ? op.gen.m11 : op.gen.m00, x);
-                    s += (double)y.re * (double)q.im - (double)y.im * (double)q.re;
-                    const Cx<R> f = one ? op.udag.m11 : op.udag.m00;
-                    vp[j] = cmul(f, x);
-                    vl[j] = cmul(f, y);
+            const unsigned cthr = op.cthr;
+            const int creg = op.creg, tgt = op.rtarget, kind = op.rkind, form = op.form;
+            const bool ok = (off & cthr) == cthr;
+            const bool plain = cthr == 0 && creg == 0;
+            R acc = R(0);
+            if (on && kind <= kRegPairReal) {
+                // tf (pending diagonal factor) commutes with the pair update, and the pair's
+                // gradient term is unchanged by a common unit-modulus factor of phi and lambda
+                reg_sweep_pair_at<R, A>(vp, vl, op.udag, creg, ok, plain, tgt, form, acc);
+                have_u = false;
+            } else if (on) {
+                const bool zgen = (form & 3) == 3;
+                const Cx<R> f0 = op.udag.m00, f1 = op.udag.m11;
+                if (kind <= kRegDiagPhase) {
+                    // target on register position tgt: the factor of index j is a scalar choice
+                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const bool one = ((j >> tgt) & 1) != 0;
+                        const Cx<R> f = csel(one, f1, f0);
+                        const Cx<R> x = vp[j], y = vl[j];
+                        const R w = zgen ? im_conj_mul(y, x) : R(0);
+                        const Cx<R> nx = cmul(f, x), ny = cmul(f, y);
+                        if (plain) {
+                            acc += one ? -w : w;
+                            vp[j] = nx;
+                            vl[j] = ny;
+                        } else {
+                            const bool c = ok && (j & creg) == creg;
+                            acc += c ? (one ? -w : w) : R(0);
+                            vp[j] = csel(c, nx, x);
+                            vl[j] = csel(c, ny, y);
+                        }
+                    });
+                } else {
+                    const u64 src = kind == kRegDiagThread ? (u64)off : base;
+                    const bool one = ((src >> tgt) & 1) != 0;
+                    Cx<R> f = csel(one, f1, f0);
+                    f = csel(cthr == 0 || ok, f, cx<R>(R(1), R(0)));
+                    if (creg == 0) {
+                        if (zgen) {
+                            if (!have_u) {
+                                u_sum = R(0);
+                                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                                    constexpr int j = decltype(jc)::value;
+                                    u_sum += im_conj_mul(vl[j], vp[j]);
+                                });
+                                have_u = true;
+                            }
+                            acc = ok ? (one ? -u_sum : u_sum) : R(0);
+                        }
+                        tf = cmul(tf, f);
+                        pending = true;
+                    } else {
+                        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                            constexpr int j = decltype(jc)::value;
+                            const bool c = (j & creg) == creg;   // scalar
+                            const Cx<R> x = vp[j], y = vl[j];
+                            if (zgen) {
+                                const R w = im_conj_mul(y, x);
+                                acc += (c && ok) ? (one ? -w : w) : R(0);
+                            }
+                            if (c) {
+                                vp[j] = cmul(f, x);
+                                vl[j] = cmul(f, y);
+                            }
+                        });
+                    }
                 }
             }
             if (op.slot >= 0) {
+                double s = (double)acc;
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
                 if (lane == 0) red[op.slot][wave] = s;
             }
         }
-#pragma unroll
-        for (int j = 0; j < A; ++j) {
-            tp[lidx[j]] = vp[j];
-            tl[lidx[j]] = vl[j];
+        if (pending) {
+            static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                vp[j] = cmul(tf, vp[j]);
+                vl[j] = cmul(tf, vl[j]);
+            });
         }
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const unsigned l = off | reg_bits<RB>(P, j);
+            tp[l] = vp[j];
+            tl[l] = vl[j];
+        });
         __syncthreads();
     }
     for (int k = threadIdx.x; k < pass.n_slots; k += T) {
@@ -827,6 +1192,26 @@ void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &g
 template void apply_gate_sequence<float>(Cx<float> *, int, const std::vector<QueuedGate> &, int, int);
 template void apply_gate_sequence<double>(Cx<double> *, int, const std::vector<QueuedGate> &, int, int);
 
+// Form code of a sweep op for the register passes: 4 * matrix form + generator (0 none, 1 X,
+// 2 Y, 3 Z). Pair ops: matrix form 1 swap, 2 real, 3 real diagonal with imaginary
+// off-diagonal, 0 anything else. Diagonal ops: 1 if m00 == 1 (only the |1> half changes),
+// else 0. The forms are exact properties of the entries, which rounding to the state
+// precision keeps.
+int sweep_form(const GateMatrix &u, int pauli, bool diagonal) {
+    const std::complex<double> m00 = u.m[0][0], m01 = u.m[0][1], m10 = u.m[1][0], m11 = u.m[1][1];
+    int f = 0;
+    if (diagonal) {
+        f = m00 == 1. ? 1 : 0;
+    } else if (m00 == 0. && m11 == 0. && m01 == 1. && m10 == 1.) {
+        f = 1;
+    } else if (m00.imag() == 0. && m01.imag() == 0. && m10.imag() == 0. && m11.imag() == 0.) {
+        f = 2;
+    } else if (m00.imag() == 0. && m11.imag() == 0. && m01.real() == 0. && m10.real() == 0.) {
+        f = 3;
+    }
+    return 4 * f + pauli;
+}
+
 template <class R>
 void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepGate> &gates, double *d_terms,
                    int max_bits, int low_bits) {
@@ -914,6 +1299,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
                 op.target = (short)g.target;
                 op.kind = (short)kOpDiagOuter;
             }
+            op.form = sweep_form(g.udag, g.pauli, op.kind != kOpPair);
             op.slot = g.pauli != 0 ? slots++ : -1;
             if (g.pauli != 0) slot_gate.push_back(i);
         }
@@ -932,9 +1318,14 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
                 for (int b = 0; b < (int)P.waves.size(); ++b) h.set(11 + b, P.waves[b]);
                 h.op_begin = P.begin;
                 h.n_ops = P.end - P.begin;
-                for (int k = P.begin; k < P.end; ++k)
-                    if (hops[pl.begin + k].kind == kOpPair)
-                        hops[pl.begin + k].target = (short)reg_position(P, hops[pl.begin + k].target);
+                for (int k = P.begin; k < P.end; ++k) {
+                    SweepOp<R> &op = hops[pl.begin + k];
+                    const RegSplit r = split_for_phase(P, op.kind, op.target, op.lctrl);
+                    op.rkind = r.kind;
+                    op.rtarget = r.target;
+                    op.creg = r.creg;
+                    op.cthr = r.cthr;
+                }
                 hphases.push_back(h);
             }
             ps.n_phases = (int)L.size();
