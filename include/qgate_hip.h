@@ -114,6 +114,18 @@ int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double p
 int qghip_qubit_processor_decohere_and_separate(qghip_qubit_processor *p, int value, double prob,
                                                 qghip_qubit_states *qs0, qghip_qubit_states *qs1,
                                                 qghip_qubit_states *qs, int local_lane);
+/* Pauli-string observable in one pass: the probability that measuring P = (x) sigma_k gives
+ * +1, where lane k carries X if bit k of x_lanes is set, Z if bit k of z_lanes is set, Y if
+ * both. Replaces what qgate/model/expand.py:66-77 (expand_pprob) lowers a PauliProb into:
+ * the basis change and CX chain of pauli_gates_diagonalizer.py:96-135, calcProbability
+ * (CPUQubitProcessor.cpp:75-125) of the last lane, and the inverse gates. FP64 result. */
+int qghip_qubit_processor_calc_pauli_probability(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                 uint64_t x_lanes, uint64_t z_lanes, double *prob);
+/* Collapse onto the eigenspace P = +1 (value 0) or -1 (value 1): psi <- (1 +- P) psi / (2 sqrt(p)),
+ * p = prob or 1 - prob. Replaces the decohere (CPUQubitProcessor.cpp:217-246) between the
+ * basis changes that expand.py:53-64 (expand_pmeasure) lowers a PauliMeasure into. */
+int qghip_qubit_processor_pauli_decohere(qghip_qubit_processor *p, int value, double prob,
+                                         qghip_qubit_states *qs, uint64_t x_lanes, uint64_t z_lanes);
 /* qubit_processor_apply_reset (glue.cpp:368-379): move the |1> half of local_lane to |0>. */
 int qghip_qubit_processor_apply_reset(qghip_qubit_processor *p, qghip_qubit_states *qs,
                                       int local_lane);

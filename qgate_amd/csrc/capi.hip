@@ -83,6 +83,11 @@ void check_lane(const qghip_qubit_states *qs, int lane) {
     if (lane < 0 || lane >= qs->n_lanes) value_error("lane out of range");
 }
 
+void check_pauli_lanes(const qghip_qubit_states *qs, uint64_t x, uint64_t z) {
+    if ((x | z) == 0) value_error("the Pauli string is the identity");
+    if (qs->n_lanes < 64 && ((x | z) >> qs->n_lanes) != 0) value_error("Pauli lane out of range");
+}
+
 // State vectors with queued gates (so that synchronize can apply them all).
 std::vector<qghip_qubit_states *> &pending_states() {
     static std::vector<qghip_qubit_states *> v;
@@ -426,6 +431,35 @@ int qghip_qubit_processor_decohere_and_separate(qghip_qubit_processor *p, int va
             launch_decohere_and_separate<R>(amps<R>(qs0), camps<R>(qs), qs->n_lanes, local_lane,
                                             value, (R)norm);
             launch_set_basis1<R>(amps<R>(qs1), value);
+        });
+    });
+}
+
+int qghip_qubit_processor_calc_pauli_probability(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                 uint64_t x_lanes, uint64_t z_lanes, double *prob) {
+    return guarded([&] {
+        if (p == nullptr || prob == nullptr) value_error("null argument");
+        check_qs(qs, p->prec);
+        check_pauli_lanes(qs, x_lanes, z_lanes);
+        flush(qs);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            *prob = calc_pauli_prob<R>(camps<R>(qs), qs->n_lanes, x_lanes, z_lanes);
+        });
+    });
+}
+
+int qghip_qubit_processor_pauli_decohere(qghip_qubit_processor *p, int value, double prob,
+                                         qghip_qubit_states *qs, uint64_t x_lanes, uint64_t z_lanes) {
+    return guarded([&] {
+        if (p == nullptr) value_error("null processor handle");
+        check_qs(qs, p->prec);
+        check_pauli_lanes(qs, x_lanes, z_lanes);
+        if (value != 0 && value != 1) value_error("value must be 0 or 1");
+        flush(qs);
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            launch_pauli_decohere<R>(amps<R>(qs), qs->n_lanes, x_lanes, z_lanes, value, prob);
         });
     });
 }

@@ -75,6 +75,12 @@ template <class R> void launch_set_zero_state(Cx<R> *a, int n_lanes);
 // P(bit `lane` == 0), FP64 accumulation; synchronises.
 template <class R> double calc_prob(const Cx<R> *a, int n_lanes, int lane);
 template <class R> void launch_decohere(Cx<R> *a, int n_lanes, int lane, int value, R norm);
+// Pauli string P over lanes x | z (X: x, Z: z, Y: both): P(measuring P gives +1) in FP64,
+// i.e. sum over pairs of |((1 + P) psi)_y|^2 / 2; synchronises.
+template <class R> double calc_pauli_prob(const Cx<R> *a, int n_lanes, u64 x, u64 z);
+// Collapse onto the eigenspace P = (-1)^value: psi <- (1 + (-1)^value P) psi / (2 sqrt(p_value)),
+// p_0 = prob, p_1 = 1 - prob.
+template <class R> void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, double prob);
 template <class R> void launch_decohere_and_separate(Cx<R> *dst, const Cx<R> *src, int n_lanes,
                                                      int lane, int value, R norm);
 template <class R> void launch_reset(Cx<R> *a, int n_lanes, int lane);

@@ -106,6 +106,24 @@ class PauliGatesDiagonalizer:
         return [op.copy() for op in self.plist + self.cxchain]
 
 
+def pauli_string(gatelist):
+    """The Pauli product of a PauliProb / PauliMeasure as [(qreg, letter)] over its
+    non-identity qubits in qreg-id order, and the qubit the expansion would measure (the last
+    of them, pauli_gates_diagonalizer.py:96-135). Per-qubit products are reduced exactly and
+    their phases dropped, as the expansion drops them for observers (expand.py:53-77)."""
+    by_qreg = {}
+    for gate in gatelist:
+        by_qreg.setdefault(gate.qreg, []).append(gate)
+    terms = []
+    for qreg in sorted(by_qreg, key=lambda q: q.id):
+        _, letter = PauliGatesDiagonalizer.reduce_lane(by_qreg[qreg])
+        if letter != 'I':
+            terms.append((qreg, letter))
+    if not terms:
+        raise RuntimeError('measurement is not z-based.')
+    return terms, terms[-1][0]
+
+
 def expand_swap(op):
     q0, q1 = op.qreglist[0], op.qreglist[1]
     return [cx(q0, q1), cx(q1, q0), cx(q0, q1)]

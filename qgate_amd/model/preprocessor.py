@@ -8,7 +8,7 @@ Follows qgate/model/preprocessor.py:9-142 and qgate/model/qreg_aggregator.py:1-7
   * one_static: all qregs of the circuit form one state vector, joined up front.
 """
 from . import ir
-from .expand import expand
+from .expand import expand, pauli_string
 from .prefs import circuit_prep, dynamic, static, one_static
 
 
@@ -58,9 +58,13 @@ class QregAggregator:
 
 class Preprocessor:
 
-    def __init__(self, **prefdict):
+    def __init__(self, fuse_pauli=False, **prefdict):
+        """fuse_pauli: keep PauliProb (and, in the static modes, PauliMeasure) as single ops
+        for a runtime that evaluates Pauli observables directly; the qubits of the string
+        join one state vector, as the expansion's CX chain would join them."""
         self.circ_prep = prefdict.get(circuit_prep, dynamic)
         self.dynamic = self.circ_prep == dynamic
+        self.fuse_pauli = fuse_pauli
         self.aggregator = QregAggregator()
         self.reset()
 
@@ -108,6 +112,20 @@ class Preprocessor:
                 raise RuntimeError('unused qreg found, {}'.format(repr(op.qreg)))
             out.append(op)
         elif isinstance(op, ir.Barrier):
+            out.append(op)
+        elif isinstance(op, (ir.PauliMeasure, ir.PauliProb)) and self.fuse_pauli and \
+                (isinstance(op, ir.PauliProb) or not self.dynamic):
+            # a Measure in dynamic mode is followed by a Separate: that case keeps the expansion
+            terms, _ = pauli_string(op.gatelist)
+            qregs = [q for q, _ in terms]
+            if len(qregs) == 1:
+                if self.aggregator.add_qreg(qregs[0]) and self.dynamic:
+                    out.append(ir.NewQreg(qregs[0]))
+            else:
+                joined = self.aggregator.aggregate(qregs)
+                if joined is not None and self.dynamic:
+                    out.append(ir.Join(joined))
+            self._refset.add(op.outref)
             out.append(op)
         elif isinstance(op, (ir.MultiQubitGate, ir.GatelistMacro, ir.PauliMeasure, ir.PauliProb)):
             out += self.preprocess_operator_list(expand(op))

@@ -14,6 +14,7 @@ runtime calls and random draws.
 import numpy.random as np_random
 
 from .. import model
+from ..model.expand import pauli_string
 
 
 class ModelExecutor:
@@ -59,6 +60,32 @@ class ModelExecutor:
         else:
             qstates.processor.decohere(result, prob, qstates, local)
 
+    def _pauli_observe(self, op):
+        """PauliProb / PauliMeasure kept whole by the preprocessor: the same value, random
+        draw and collapse as their expansion (basis change, CX chain, Prob / Measure of the
+        last qubit of the string, inverse gates; qgate/model/expand.py:53-77), in one pass."""
+        terms, op_qreg = pauli_string(op.gatelist)
+        lanes = self._qhandler.lanes
+        qstates = lanes[op_qreg].qstates
+        x_lanes = z_lanes = 0
+        for qreg, letter in terms:
+            lane = lanes[qreg]
+            assert lane.qstates is qstates, 'Pauli observable spans several state vectors.'
+            if letter in 'XY':
+                x_lanes |= 1 << lane.local
+            if letter in 'YZ':
+                z_lanes |= 1 << lane.local
+        proc = qstates.processor
+        if isinstance(op, model.PauliProb):
+            self._value_store.set(op.outref, proc.calc_pauli_probability(qstates, x_lanes, z_lanes))
+            return
+        randnum = np_random.random_sample()
+        prob = proc.calc_pauli_probability(qstates, x_lanes, z_lanes)
+        result = 0 if randnum < prob else 1
+        self._value_store.set(op.outref, result)
+        qstates.set_lane_state(lanes[op_qreg].local, result)
+        proc.pauli_decohere(result, prob, qstates, x_lanes, z_lanes)
+
     def _dispatch(self, op):
         lanes = self._qhandler.lanes
         if isinstance(op, model.Gate):
@@ -80,6 +107,8 @@ class ModelExecutor:
             lane = lanes[op.qreg]
             self._value_store.set(op.outref,
                                   lane.qstates.processor.calc_probability(lane.qstates, lane.local))
+        elif isinstance(op, (model.PauliProb, model.PauliMeasure)):
+            self._pauli_observe(op)
         elif isinstance(op, model.Reset):
             lane = lanes[op.qreg]
             bit = lane.qstates.get_lane_state(lane.local)

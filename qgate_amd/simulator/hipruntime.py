@@ -23,6 +23,10 @@ from ..model import gate_type as gtype
 
 this = sys.modules[__name__]
 
+# The processor evaluates PauliProb / PauliMeasure in one pass (calc_pauli_probability,
+# pauli_decohere); the simulator then keeps them whole instead of expanding them into gates.
+PAULI_OBSERVABLES = True
+
 _OK, _ERROR, _EVALUE = 0, -1, -2
 FP64, FP32 = 0, 1
 MATHOP_NULL, MATHOP_PROB = 0, 1
@@ -75,6 +79,10 @@ def lib():
         'qghip_qubit_processor_decohere': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double, H, ctypes.c_int]),
         'qghip_qubit_processor_decohere_and_separate': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double, H, H, H, ctypes.c_int]),
         'qghip_qubit_processor_apply_reset': (ctypes.c_int, [H, H, ctypes.c_int]),
+        'qghip_qubit_processor_calc_pauli_probability': (ctypes.c_int, [H, H, ctypes.c_uint64, ctypes.c_uint64,
+                                                                        _c_double_p]),
+        'qghip_qubit_processor_pauli_decohere': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double, H,
+                                                                ctypes.c_uint64, ctypes.c_uint64]),
         'qghip_qubit_processor_apply_gate': (ctypes.c_int, [H, ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_int, H, ctypes.c_int]),
         'qghip_qubit_processor_apply_controlled_gate': (ctypes.c_int, [H, ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_int, H, _c_int_p, ctypes.c_int, ctypes.c_int]),
         'qghip_qubit_processor_apply_matrix': (ctypes.c_int, [H, _c_double_p, H, _c_int_p, ctypes.c_int, ctypes.c_int]),
@@ -278,6 +286,17 @@ class HipQubitProcessor:
 
     def apply_reset(self, qstates, local_lane):
         check(lib().qghip_qubit_processor_apply_reset(self.ptr, qstates.ptr, local_lane))
+
+    def calc_pauli_probability(self, qstates, x_lanes, z_lanes):
+        """P(+1) of the Pauli string with X on x_lanes, Z on z_lanes, Y on both (lane masks)."""
+        out = ctypes.c_double()
+        check(lib().qghip_qubit_processor_calc_pauli_probability(self.ptr, qstates.ptr, x_lanes, z_lanes,
+                                                                  ctypes.byref(out)))
+        return out.value
+
+    def pauli_decohere(self, value, prob, qstates, x_lanes, z_lanes):
+        """Collapse onto the eigenspace of the Pauli string with eigenvalue (-1)^value."""
+        check(lib().qghip_qubit_processor_pauli_decohere(self.ptr, value, prob, qstates.ptr, x_lanes, z_lanes))
 
     def apply_gate(self, gate_type, adjoint, qstates, local_lane):
         args = gate_type.args
