@@ -23,6 +23,9 @@ batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), ea
 own parameters and observable, <H> and gradients in one graph replay, against a loop of
 single-circuit evaluations.
 
+pauli_observable_30q: P(+1) of a 30-factor X/Y/Z string on the same state in one read pass,
+against the reference's lowering of that PauliProb into gates (basis change, CX chain, P(0)).
+
 adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qubit, 12-layer
 RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
 once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
@@ -239,6 +242,53 @@ def batch_timing(n_circuits=4096, layers=10, seed=5, loop_sample=512):
             'groups': [{k: g[k] for k in ('qubits', 'items', 'tile_bits', 'threads', 'grid_per_cu')} for g in tuning]}
 
 
+def pauli_timing(r, n, reps=5):
+    """One Pauli-string observable on the n-qubit complex64 state of runner r: X on lanes
+    0, 3, 6, ..., Y on 1, 4, 7, ..., Z on 2, 5, 8, ... P(+1) in one read pass
+    (calc_pauli_probability; algorithmic bytes 8 * 2^n) against the reference's lowering of
+    the same PauliProb: basis change, CX chain, P(last lane = 0), inverse gates."""
+    from qgate_amd.model import gate_type as gt
+    x = z = 0
+    for lane in range(n):
+        if lane % 3 in (0, 1):
+            x |= 1 << lane
+        if lane % 3 in (1, 2):
+            z |= 1 << lane
+    fused = r.proc.calc_pauli_probability(r.qs, x, z)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fused = r.proc.calc_pauli_probability(r.qs, x, z)
+    t_fused = (time.perf_counter() - t0) / reps
+
+    def expanded():
+        basis = [(lane, lane % 3) for lane in range(n)]
+        for lane, k in basis:   # X -> H, Y -> SH^dagger (the lowering's first gates)
+            if k == 0:
+                r.proc.apply_gate(gt.H(), False, r.qs, lane)
+            elif k == 1:
+                r.proc.apply_gate(gt.SH(), True, r.qs, lane)
+        for lane in range(n - 1):
+            r.proc.apply_controlled_gate(gt.X(), False, r.qs, [lane], lane + 1)
+        p = r.proc.calc_probability(r.qs, n - 1)
+        for lane in reversed(range(n - 1)):
+            r.proc.apply_controlled_gate(gt.X(), False, r.qs, [lane], lane + 1)
+        for lane, k in reversed(basis):
+            if k == 0:
+                r.proc.apply_gate(gt.H(), False, r.qs, lane)
+            elif k == 1:
+                r.proc.apply_gate(gt.SH(), False, r.qs, lane)
+        r.sync()
+        return p
+
+    t1 = time.perf_counter()
+    p_exp = expanded()
+    t_exp = time.perf_counter() - t1
+    return {'qubits': n, 'dtype': 'complex64', 'string': 'X,Y,Z on every third qubit (30 factors)',
+            'fused_ms': round(t_fused * 1e3, 3), 'fused_GBs': round(8 * (1 << n) / t_fused / 1e9, 1),
+            'expanded_ms': round(t_exp * 1e3, 2), 'speedup': round(t_exp / t_fused, 1),
+            'prob': fused, 'abs_diff_vs_expanded': abs(fused - p_exp)}
+
+
 def large_state_timing(n=34):
     """Config of the large single-GPU state: 34 qubits complex64 (128 GiB). H on every qubit,
     then per-target timing of an uncontrolled gate (HIP events, 2 launches per target), P(0)
@@ -346,6 +396,10 @@ def main():
     r.proc.set_fusion('off')
     print('[bench] fused step {:.3f} s'.format(fused_s), file=sys.stderr, flush=True)
 
+    # a Pauli observable on the circuit's output state (before the measurements collapse it)
+    pauli = pauli_timing(r, n)
+    print('[bench] pauli observable: {}'.format(pauli), file=sys.stderr, flush=True)
+
     # end-of-circuit work of the workload on the same state: pool + 2^20 draws, measures
     from qgate_amd.simulator.lanes import Lane
     getter = hipruntime.create_qubits_states_getter(np.float32)
@@ -398,6 +452,7 @@ def main():
         'fused_tile': {'ms_per_step': round(fused_s * 1e3, 2), 'gates_per_s': round(len(gates) / fused_s, 1),
                        'gate_bytes_per_s_GBs': round(step_bytes / fused_s / 1e9, 1),
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
+        'pauli_observable_30q': pauli,
         'adjoint_24q': adjoint,
         'batched_adjoint': batched,
         'large_state_34q': large,
