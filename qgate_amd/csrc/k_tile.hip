@@ -1046,27 +1046,46 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
     lds_to_tile<R, T, upt>(lam, tl, base, pass.pos, M, pass.low);
 }
 
+// Sum of row[0..n) over a workgroup of T threads in a fixed order (valid in thread 0). Thread
+// t adds row[t + k T] into 8 interleaved accumulators, so 8 independent loads are in flight,
+// and the accumulators and then the threads are combined in a fixed tree.
+template <int T>
+__device__ __forceinline__ double row_sum(const double *__restrict__ row, u64 n, double *scratch) {
+    double a0 = 0., a1 = 0., a2 = 0., a3 = 0., a4 = 0., a5 = 0., a6 = 0., a7 = 0., tail = 0.;
+    u64 i = threadIdx.x;
+    for (; i + 7 * (u64)T < n; i += 8 * (u64)T) {
+        a0 += row[i];
+        a1 += row[i + T];
+        a2 += row[i + 2 * T];
+        a3 += row[i + 3 * T];
+        a4 += row[i + 4 * T];
+        a5 += row[i + 5 * T];
+        a6 += row[i + 6 * T];
+        a7 += row[i + 7 * T];
+    }
+    for (; i < n; i += T) tail += row[i];
+    const double s = (((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7))) + tail;
+    return block_sum_tile<T>(s, scratch);
+}
+
+constexpr int kRowThreads = 1024;
+
 // terms[slot_gate[r]] = sum over tiles of partial[r * n_tiles + tile], fixed order; one
-// wavefront per slot row.
-__global__ __launch_bounds__(64) void k_slot_rows(const double *__restrict__ partial, u64 n_tiles,
-                                                  const int *__restrict__ slot_gate, double *__restrict__ terms) {
-    const double *row = partial + (u64)blockIdx.x * n_tiles;
-    double s = 0.;
-    for (u64 i = threadIdx.x; i < n_tiles; i += 64) s += row[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+// workgroup per slot row.
+__global__ __launch_bounds__(kRowThreads) void k_slot_rows(const double *__restrict__ partial, u64 n_tiles,
+                                                          const int *__restrict__ slot_gate,
+                                                          double *__restrict__ terms) {
+    __shared__ double scratch[kRowThreads / 64];
+    const double s = row_sum<kRowThreads>(partial + (u64)blockIdx.x * n_tiles, n_tiles, scratch);
     if (threadIdx.x == 0) terms[slot_gate[blockIdx.x]] = s;
 }
 
-// terms[slot] (+)= sum over tiles of partial[slot * n_tiles + tile], fixed order; one
-// workgroup per slot.
-__global__ __launch_bounds__(256) void k_sum_slots(const double *__restrict__ partial, u64 n_tiles,
-                                                   double *__restrict__ terms) {
-    __shared__ double scratch[4];
-    const double *row = partial + (u64)blockIdx.x * n_tiles;
-    double s = 0.;
-    for (u64 i = threadIdx.x; i < n_tiles; i += 256) s += row[i];
-    s = block_sum_tile<256>(s, scratch);
+// terms[slot] = sum over tiles of partial[slot * n_tiles + tile], fixed order; one workgroup
+// per slot.
+__global__ __launch_bounds__(kRowThreads) void k_sum_slots(const double *__restrict__ partial, u64 n_tiles,
+                                                          double *__restrict__ terms) {
+    __shared__ double scratch[kRowThreads / 64];
+    const double s = row_sum<kRowThreads>(partial + (u64)blockIdx.x * n_tiles, n_tiles, scratch);
     if (threadIdx.x == 0) terms[blockIdx.x] = s;
 }
 
@@ -1370,7 +1389,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
                                    d_ops, d_part, n_tiles, first);
         }
         if (ps.n_slots > 0)   // terms of this pass's slots land at d_terms[gate index]
-            hipLaunchKernelGGL(k_slot_rows, (unsigned)ps.n_slots, 64, 0, s, d_part, n_tiles, d_sg + pass_slot_begin[pi],
+            hipLaunchKernelGGL(k_slot_rows, (unsigned)ps.n_slots, kRowThreads, 0, s, d_part, n_tiles, d_sg + pass_slot_begin[pi],
                                d_terms);
         e = hipGetLastError();
     }
@@ -1491,7 +1510,7 @@ double pauli_expectation(const Cx<R> *psi, Cx<R> *lam, int n_lanes, const std::v
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_sum_slots, 1, 256, 0, s, d_part, n_tiles_last, d_part + n_tiles_last);
+        hipLaunchKernelGGL(k_sum_slots, 1, kRowThreads, 0, s, d_part, n_tiles_last, d_part + n_tiles_last);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
