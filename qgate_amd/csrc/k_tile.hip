@@ -787,6 +787,31 @@ constexpr int kMaxSweepSlots = 96;
 
 typedef PhaseBits SweepPhase;   // op_begin relative to the pass
 
+// One DPP step on a double: lane l receives v of the lane that control CTRL names, in the rows
+// enabled by ROWS (other rows receive 0). DPP moves run on the vector ALU, with no LDS
+// round trip.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_double(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum of v over the 64 lanes of a wavefront in a fixed order, valid in lane 63: butterflies
+// within quads (quad_perm [1,0,3,2], [2,3,0,1]) and rows (row_ror 4, 8) give every lane its
+// 16-lane row sum; row_bcast 15 then adds row 0 into row 1 and row 2 into row 3, and
+// row_bcast 31 adds rows 0+1 into row 3.
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+    v += dpp_double<0xb1, 0xf>(v);
+    v += dpp_double<0x4e, 0xf>(v);
+    v += dpp_double<0x124, 0xf>(v);
+    v += dpp_double<0x128, 0xf>(v);
+    v += dpp_double<0x142, 0xa>(v);
+    v += dpp_double<0x143, 0xc>(v);
+    return v;
+}
+
 // Register sweep pair update on register bit P: the gradient term Im<lambda|G|phi> over the
 // pair for generator G (0 none, 1 X, 2 Y, 3 Z), accumulated in `acc` (state precision; the
 // caller widens each thread's sum to FP64), then U^dagger of form F on phi and lambda.
@@ -1015,10 +1040,8 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
                 }
             }
             if (op.slot >= 0) {
-                double s = (double)acc;
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-                if (lane == 0) red[op.slot][wave] = s;
+                const double s = wave_sum_lane63((double)acc);
+                if (lane == 63) red[op.slot][wave] = s;
             }
         }
         if (pending) {
