@@ -124,36 +124,41 @@ void get_states(void *host_out, int mathop, const std::vector<StateView> &qs, u6
     hipStream_t s = stream();
     const size_t item = mathop == 0 ? sizeof(Cx<R>) : sizeof(R);
     if (mathop == 0 && step == 1 && empty_mask == 0 && is_identity_view(qs)) {
-        // external order == local order: one device-to-host copy
+        // external order == local order: the state itself goes through the pinned staging
         const Cx<R> *a = static_cast<const Cx<R> *>(qs[0].ptr);
-        QG_HIP(hipMemcpyAsync(host_out, a + start, item * (size_t)n_states, hipMemcpyDeviceToHost, s));
-        QG_HIP(hipStreamSynchronize(s));
+        copy_to_host(host_out, a + start, item * (size_t)n_states);
         return;
     }
+    // gathered in chunks of one staging buffer into two device buffers used in turn: the
+    // kernel for chunk k + 1 runs while chunk k is copied out on the host
     const ViewArgs g = make_view(qs);
-    const i64 chunk = i64(1) << 26;
+    const i64 chunk = (i64)(kStageBytes / item);
     const i64 staged = std::min<i64>(chunk, n_states);
-    void *d_out = dev_alloc(item * (size_t)staged);
+    const size_t dbytes = item * (size_t)staged;
+    char *d_out = static_cast<char *>(dev_alloc(2 * dbytes));
     try {
-        for (i64 done = 0; done < n_states; done += chunk) {
+        HostStager stager;
+        int k = 0;
+        for (i64 done = 0; done < n_states; done += chunk, k ^= 1) {
             const i64 cnt = std::min<i64>(chunk, n_states - done);
             const i64 st = start + done * step;
+            char *d = d_out + k * dbytes;
             if (mathop == 0)
-                hipLaunchKernelGGL((k_get_states<R, 0>), blocks_for(cnt), kBlock, 0, s, d_out, g,
+                hipLaunchKernelGGL((k_get_states<R, 0>), blocks_for(cnt), kBlock, 0, s, d, g,
                                    empty_mask, st, step, (u64)cnt);
             else
-                hipLaunchKernelGGL((k_get_states<R, 1>), blocks_for(cnt), kBlock, 0, s, d_out, g,
+                hipLaunchKernelGGL((k_get_states<R, 1>), blocks_for(cnt), kBlock, 0, s, d, g,
                                    empty_mask, st, step, (u64)cnt);
             QG_HIP(hipGetLastError());
-            QG_HIP(hipMemcpyAsync(static_cast<char *>(host_out) + item * done, d_out, item * cnt,
-                                  hipMemcpyDeviceToHost, s));
-            QG_HIP(hipStreamSynchronize(s));
+            stager.push(static_cast<char *>(host_out) + item * done, d, item * (size_t)cnt);
         }
+        stager.finish();
     } catch (...) {
-        dev_free(d_out, item * (size_t)staged);
+        hipStreamSynchronize(s);
+        dev_free(d_out, 2 * dbytes);
         throw;
     }
-    dev_free(d_out, item * (size_t)staged);
+    dev_free(d_out, 2 * dbytes);
 }
 
 template <class R>

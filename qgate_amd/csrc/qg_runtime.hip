@@ -2,9 +2,12 @@
 // Counterpart of the reference device setup and memory store
 // (qgate/simulator/src/CUDADevice.cpp:60-153, MultiDeviceMemoryStore.cpp:164-269), for one GPU:
 // no chunking across devices, and no device-wide synchronisation on allocate/free.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "qg_runtime.hpp"
@@ -99,11 +102,114 @@ void runtime_init(int device) {
     rt.initialized = true;
 }
 
+namespace {
+
+struct StageBuffers {
+    char *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+StageBuffers &stage_buffers() {
+    static StageBuffers sb;
+    return sb;
+}
+
+StageBuffers &staging() {
+    stream();
+    StageBuffers &sb = stage_buffers();
+    if (sb.buf[0] == nullptr) {
+        for (int b = 0; b < 2; ++b) {
+            QG_HIP(hipHostMalloc(reinterpret_cast<void **>(&sb.buf[b]), kStageBytes, hipHostMallocDefault));
+            QG_HIP(hipEventCreateWithFlags(&sb.ev[b], hipEventDisableTiming));
+        }
+    }
+    return sb;
+}
+
+void release_staging() {
+    StageBuffers &sb = stage_buffers();
+    for (int b = 0; b < 2; ++b) {
+        if (sb.buf[b] != nullptr) hipHostFree(sb.buf[b]);
+        if (sb.ev[b] != nullptr) hipEventDestroy(sb.ev[b]);
+        sb.buf[b] = nullptr;
+        sb.ev[b] = nullptr;
+    }
+}
+
+// memcpy split over up to 8 host threads (first touch of fresh pages included)
+void parallel_memcpy(char *dst, const char *src, size_t bytes) {
+    constexpr size_t kMinPerThread = size_t(4) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t n = std::min<size_t>({8, hw, std::max<size_t>(1, bytes / kMinPerThread)});
+    if (n <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes / n + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < n; ++i) {
+        const size_t off = i * part;
+        if (off >= bytes) break;
+        const size_t len = std::min(part, bytes - off);
+        th.emplace_back([=] { std::memcpy(dst + off, src + off, len); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (std::thread &t : th) t.join();
+}
+
+}  // namespace
+
+HostStager::HostStager() { staging(); }
+
+HostStager::~HostStager() {
+    // a stager abandoned by an exception still waits for its copies into the staging buffers
+    for (int b = 0; b < 2; ++b)
+        if (slot_[b].active) hipEventSynchronize(stage_buffers().ev[b]);
+}
+
+void HostStager::drain(int b) {
+    StageBuffers &sb = stage_buffers();
+    QG_HIP(hipEventSynchronize(sb.ev[b]));
+    slot_[b].active = false;
+    parallel_memcpy(slot_[b].dst, sb.buf[b], slot_[b].bytes);
+}
+
+void HostStager::push(void *dst, const void *src, size_t bytes) {
+    QG_REQUIRE(bytes <= kStageBytes, "staged chunk of %zu bytes exceeds the staging buffer", bytes);
+    StageBuffers &sb = staging();
+    const int b = next_;
+    if (slot_[b].active) drain(b);
+    QG_HIP(hipMemcpyAsync(sb.buf[b], src, bytes, hipMemcpyDeviceToHost, stream()));
+    QG_HIP(hipEventRecord(sb.ev[b], stream()));
+    slot_[b].dst = static_cast<char *>(dst);
+    slot_[b].bytes = bytes;
+    slot_[b].active = true;
+    next_ ^= 1;
+}
+
+void HostStager::finish() {
+    // the older pending chunk sits in buffer next_, the newer one in next_ ^ 1
+    for (int k = 0; k < 2; ++k) {
+        const int b = next_ ^ k;
+        if (slot_[b].active) drain(b);
+    }
+}
+
+void copy_to_host(void *dst, const void *src, size_t bytes) {
+    HostStager st;
+    const char *s = static_cast<const char *>(src);
+    char *d = static_cast<char *>(dst);
+    for (size_t done = 0; done < bytes; done += kStageBytes)
+        st.push(d + done, s + done, std::min(kStageBytes, bytes - done));
+    st.finish();
+}
+
 void runtime_clear() {
     Runtime &rt = runtime();
     std::lock_guard<std::recursive_mutex> lk(rt.mtx);
     if (!rt.initialized) return;
     hipStreamSynchronize(rt.stream);
+    release_staging();
     release_cache();
     hipFree(rt.d_partials);
     hipHostFree(rt.h_pinned);

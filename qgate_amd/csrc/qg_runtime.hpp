@@ -63,4 +63,32 @@ inline unsigned grid_for(u64 count, int block) { return (unsigned)((count + bloc
 bool use_register_tiles();
 void set_register_tiles(bool on);
 
+// Device-to-host readout through pinned staging. Chunks of up to kStageBytes go by DMA into
+// one of two pinned host buffers (hipHostMalloc, allocated on first use) while host threads
+// copy the previous chunk out to the caller's (pageable) memory.
+constexpr size_t kStageBytes = size_t(64) << 20;
+
+class HostStager {
+  public:
+    HostStager();
+    ~HostStager();
+    // Queue a copy of `bytes` (<= kStageBytes) from device memory `src`, ordered on the
+    // library stream after the work already queued, to host memory `dst`.
+    void push(void *dst, const void *src, size_t bytes);
+    // Complete every queued copy.
+    void finish();
+
+  private:
+    void drain(int b);
+    struct Slot {
+        char *dst = nullptr;
+        size_t bytes = 0;
+        bool active = false;
+    } slot_[2];
+    int next_ = 0;
+};
+
+// Copy `bytes` of device memory to pageable host memory through the stager; synchronises.
+void copy_to_host(void *dst, const void *src, size_t bytes);
+
 }  // namespace qg
