@@ -23,6 +23,9 @@ batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), ea
 own parameters and observable, <H> and gradients in one graph replay, against a loop of
 single-circuit evaluations.
 
+readout_30q: the whole state (8 GiB of amplitudes) and its probabilities (4 GiB) copied to
+host memory through pinned double-buffered staging (best of two).
+
 pauli_observable_30q: P(+1) of a 30-factor X/Y/Z string on the same state in one read pass,
 against the reference's lowering of that PauliProb into gates (basis change, CX chain, P(0)).
 
@@ -400,7 +403,8 @@ def main():
     pauli = pauli_timing(r, n)
     print('[bench] pauli observable: {}'.format(pauli), file=sys.stderr, flush=True)
 
-    # end-of-circuit work of the workload on the same state: pool + 2^20 draws, measures
+    # end-of-circuit work of the workload on the same state: readout, pool + 2^20 draws, measures
+    from qgate_amd.simulator import qubits as qmod
     from qgate_amd.simulator.lanes import Lane
     getter = hipruntime.create_qubits_states_getter(np.float32)
     lanes = []
@@ -408,6 +412,18 @@ def main():
         ln = Lane(None, i)
         ln.set_external(i)
         lanes.append(ln)
+    readout = {}
+    for name, op, dt in (('amplitudes', qmod.null, np.complex64), ('probabilities', qmod.abs2, np.float32)):
+        out = np.empty(1 << n, dt)
+        ts = []
+        for _ in range(2):
+            tr0 = time.perf_counter()
+            getter.get_states(out, 0, op, [(r.qs, lanes)], [], 1 << n, 0, 1)
+            ts.append(time.perf_counter() - tr0)
+        readout[name] = {'GiB': out.nbytes / 2 ** 30, 'ms': round(min(ts) * 1e3, 1),
+                         'GBs': round(out.nbytes / min(ts) / 1e9, 1)}
+        del out
+    print('[bench] readout: {}'.format(readout), file=sys.stderr, flush=True)
     u = np.random.RandomState(1002).random_sample(1 << 20)
     t2 = time.perf_counter()
     pool = getter.create_sampling_pool(list(range(n)), n, 0, [(r.qs, lanes)], [])
@@ -453,6 +469,7 @@ def main():
                        'gate_bytes_per_s_GBs': round(step_bytes / fused_s / 1e9, 1),
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
         'pauli_observable_30q': pauli,
+        'readout_30q': readout,
         'adjoint_24q': adjoint,
         'batched_adjoint': batched,
         'large_state_34q': large,
