@@ -29,6 +29,9 @@ host memory through pinned double-buffered staging (best of two).
 pauli_observable_30q: P(+1) of a 30-factor X/Y/Z string on the same state in one read pass,
 against the reference's lowering of that PauliProb into gates (basis change, CX chain, P(0)).
 
+openqasm_qft_30q: the 30-qubit QFT as OpenQASM text: host translate + load time, then the
+run on the GPU (complex64, fusion on, first run of a fresh state).
+
 adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qubit, 12-layer
 RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
 once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
@@ -292,6 +295,39 @@ def pauli_timing(r, n, reps=5):
             'prob': fused, 'abs_diff_vs_expanded': abs(fused - p_exp)}
 
 
+def qasm_qft_timing(n=30):
+    """The QFT on n qubits written as OpenQASM 2.0 (h, cu1 and swap lines after an h layer on
+    every qubit): translate + load time on the host, then the circuit on the HIP runtime in
+    complex64 with fusion (tile passes), state resident in HBM."""
+    from qgate_amd import openqasm
+    from qgate_amd.simulator import hipruntime
+    from qgate_amd.simulator.simulator import Simulator
+    lines = ['OPENQASM 2.0;', 'include "qelib1.inc";', 'qreg q[{}];'.format(n), 'h q;']
+    for j in reversed(range(n)):
+        lines.append('h q[{}];'.format(j))
+        for k in reversed(range(j)):
+            lines.append('cu1(pi/{}) q[{}], q[{}];'.format(2 ** (j - k), k, j))
+    for j in range(n // 2):
+        lines.append('swap q[{}], q[{}];'.format(j, n - 1 - j))
+    text = '\n'.join(lines) + '\n'
+    t0 = time.perf_counter()
+    mod = openqasm.load_circuit(text)
+    t1 = time.perf_counter()
+    hipruntime.set_fusion('tile')
+    sim = Simulator(hipruntime, circuit_prep='one_static', dtype=np.float32)
+    sim.qubits.set_ordering(list(mod.q))
+    try:
+        sim.run(mod.circuit)   # first run: allocation and |0> included
+        t2 = time.perf_counter()
+        p0 = sim.qubits.calc_probability(mod.q[0])
+        t3 = time.perf_counter()
+    finally:
+        hipruntime.set_fusion('off')
+        sim.terminate()
+    return {'qubits': n, 'qasm_lines': len(lines), 'load_ms': round((t1 - t0) * 1e3, 2),
+            'run_ms': round((t2 - t1) * 1e3, 2), 'p0_qubit0': p0, 'p0_ms': round((t3 - t2) * 1e3, 2)}
+
+
 def large_state_timing(n=34):
     """Config of the large single-GPU state: 34 qubits complex64 (128 GiB). H on every qubit,
     then per-target timing of an uncontrolled gate (HIP events, 2 launches per target), P(0)
@@ -450,6 +486,8 @@ def main():
     print('[bench] batch: {}'.format(batched), file=sys.stderr, flush=True)
     large = large_state_timing() if not args.no_large else None
     print('[bench] large state: {}'.format(large), file=sys.stderr, flush=True)
+    qasm = qasm_qft_timing()
+    print('[bench] openqasm qft: {}'.format(qasm), file=sys.stderr, flush=True)
 
     result = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'GB/s', 'n_gpus': 1,
@@ -473,6 +511,7 @@ def main():
         'adjoint_24q': adjoint,
         'batched_adjoint': batched,
         'large_state_34q': large,
+        'openqasm_qft_30q': qasm,
         'roofline': roof,
     }
     if not args.no_cpu:

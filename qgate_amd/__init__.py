@@ -3,6 +3,7 @@ from . import model
 from . import simulator
 from .model import prefs
 from . import script
+from . import openqasm
 
 
 def dump(obj, file=None, number_format=None):
