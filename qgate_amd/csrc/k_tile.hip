@@ -928,6 +928,36 @@ __device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A]
     }
 }
 
+// Registers of one state in phase layout (off, P) <-> LDS tile t.
+template <class R, int A, int RB>
+__device__ __forceinline__ void regs_from_lds(Cx<R> (&v)[A], const Cx<R> *t, unsigned off, const PhaseBits &P) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = t[off | reg_bits<RB>(P, j)];
+    });
+}
+
+template <class R, int A, int RB>
+__device__ __forceinline__ void regs_to_lds(Cx<R> *t, const Cx<R> (&v)[A], unsigned off, const PhaseBits &P) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        t[off | reg_bits<RB>(P, j)] = v[j];
+    });
+}
+
+template <int WB>
+__device__ __forceinline__ unsigned phase_offset(const PhaseBits &P, unsigned lane, unsigned wave) {
+    unsigned off = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb(k);
+#pragma unroll
+    for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
+    return off;
+}
+
+// phi and lambda tiles in registers, one LDS tile each for the layout changes. Every LDS
+// location written at the end of a phase is the one the same thread read at its start, so
+// only the hand-overs between layouts need barriers.
 template <class R, int M, int T>
 __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, Cx<R> *__restrict__ lam,
                                                       const SweepPass pass, const SweepOp<R> *__restrict__ ops,
@@ -941,24 +971,27 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
     const u64 tile = first + blockIdx.x;
     const u64 base = tile_base(tile, pass.pos, M);
     constexpr int upt = (1 << (M - lw)) / T;   // units per thread
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Cx<R> vp[A], vl[A];
+    SweepPhase P = phases[pass.phase_begin];
+    unsigned off = phase_offset<WB>(P, lane, wave);
     tile_to_lds<R, T, upt>(tp, phi, base, pass.pos, M, pass.low);
     tile_to_lds<R, T, upt>(tl, lam, base, pass.pos, M, pass.low);
     __syncthreads();
-    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    Cx<R> vp[A], vl[A];
+    regs_from_lds<R, A, RB>(vp, tp, off, P);
+    regs_from_lds<R, A, RB>(vl, tl, off, P);
     for (int ph = 0; ph < pass.n_phases; ++ph) {
-        const SweepPhase P = phases[pass.phase_begin + ph];
-        unsigned off = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) off |= ((lane >> k) & 1u) << P.lb(k);
-#pragma unroll
-        for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
-        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            const unsigned l = off | reg_bits<RB>(P, j);
-            vp[j] = tp[l];
-            vl[j] = tl[l];
-        });
+        if (ph > 0) {   // registers from phase ph - 1's layout to phase ph's
+            const SweepPhase Q = phases[pass.phase_begin + ph];
+            const unsigned qoff = phase_offset<WB>(Q, lane, wave);
+            regs_to_lds<R, A, RB>(tp, vp, off, P);
+            regs_to_lds<R, A, RB>(tl, vl, off, P);
+            __syncthreads();
+            regs_from_lds<R, A, RB>(vp, tp, qoff, Q);
+            regs_from_lds<R, A, RB>(vl, tl, qoff, Q);
+            P = Q;
+            off = qoff;
+        }
         // As in the forward register pass, diagonal gates on no register bit are one factor per
         // thread (tf), applied to phi and lambda at the end of the phase. Diagonal gates
         // leave w_j = Im(conj(lambda_j) phi_j) unchanged (|factor| = 1), so the generator-Z
@@ -1051,22 +1084,18 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
                 vl[j] = cmul(tf, vl[j]);
             });
         }
-        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            const unsigned l = off | reg_bits<RB>(P, j);
-            tp[l] = vp[j];
-            tl[l] = vl[j];
-        });
-        __syncthreads();
     }
+    regs_to_lds<R, A, RB>(tp, vp, off, P);
+    regs_to_lds<R, A, RB>(tl, vl, off, P);
+    __syncthreads();
+    lds_to_tile<R, T, upt>(phi, tp, base, pass.pos, M, pass.low);
+    lds_to_tile<R, T, upt>(lam, tl, base, pass.pos, M, pass.low);
     for (int k = threadIdx.x; k < pass.n_slots; k += T) {
         double t = 0.;
 #pragma unroll
         for (int w = 0; w < NW; ++w) t += red[k][w];
         partial[(u64)k * n_tiles + tile] = t;
     }
-    lds_to_tile<R, T, upt>(phi, tp, base, pass.pos, M, pass.low);
-    lds_to_tile<R, T, upt>(lam, tl, base, pass.pos, M, pass.low);
 }
 
 // Sum of row[0..n) over a workgroup of T threads in a fixed order (valid in thread 0). Thread
@@ -1192,14 +1221,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pauli(const Cx<R> *__rest
 
 template <class R>
 struct SweepCfg;
+// Sweep tiles: register passes (RegBits, RegThreads) and LDS passes for smaller tiles or with
+// register tiles off (LdsBits), both holding a phi and a lambda tile. 12-bit complex64 tiles
+// (512 threads, 64 KiB of LDS) cut the 24-qubit ansatz's backward from 38 passes to 33 but
+// measured no faster: the sweep is bound by its per-gate instruction work, not by passes.
 template <>
 struct SweepCfg<float> {
-    static constexpr int MaxBits = 11;   // two 2^11 x 8 B tiles = 32 KiB of LDS
+    static constexpr int RegBits = 11, RegThreads = 256;   // 8 amplitudes per state and thread
+    static constexpr int LdsBits = 11;                     // two 2^11 x 8 B tiles = 32 KiB
     static constexpr int Low = 5;
 };
 template <>
 struct SweepCfg<double> {
-    static constexpr int MaxBits = 10;   // two 2^10 x 16 B tiles
+    static constexpr int RegBits = 10, RegThreads = 256;   // 4 amplitudes per state and thread
+    static constexpr int LdsBits = 10;                     // two 2^10 x 16 B tiles = 32 KiB
     static constexpr int Low = 4;
 };
 
@@ -1258,7 +1293,9 @@ template <class R>
 void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepGate> &gates, double *d_terms,
                    int max_bits, int low_bits) {
     typedef SweepCfg<R> C;
-    const int mb = std::min(max_bits > 0 ? std::min(max_bits, C::MaxBits) : C::MaxBits, n_lanes);
+    const bool regs = use_register_tiles();
+    const int cap = regs ? C::RegBits : C::LdsBits;
+    const int mb = std::min(max_bits > 0 ? std::min(max_bits, cap) : cap, n_lanes);
     const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
     // plan the passes: like apply_gate_sequence, with at most kMaxSweepSlots rotations a pass
     struct Plan {
@@ -1289,8 +1326,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     }
     if (cur.end > cur.begin) plans.push_back(cur);
 
-    constexpr int A = (1 << C::MaxBits) / kTileThreads;
-    const bool regs = use_register_tiles();
+    constexpr int A = (1 << C::RegBits) / C::RegThreads;
     // device tables: ops (pass by pass), phases of register passes, slot -> gate index
     std::vector<SweepOp<R>> hops(std::max<size_t>(1, gates.size()));
     std::vector<SweepPhase> hphases;
@@ -1346,12 +1382,13 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
             if (g.pauli != 0) slot_gate.push_back(i);
         }
         ps.n_slots = slots;
-        const bool use_regs = regs && ps.m == C::MaxBits;
+        const bool use_regs = regs && ps.m == C::RegBits;
+        QG_REQUIRE(use_regs || ps.m <= C::LdsBits, "adjoint sweep: tile of %d bits", ps.m);
         if (use_regs) {
             std::vector<int> pair(ps.n_ops);
             for (int k = 0; k < ps.n_ops; ++k)
                 pair[k] = hops[pl.begin + k].kind == kOpPair ? hops[pl.begin + k].target : -1;
-            const std::vector<PhaseLayout> L = plan_phase_layouts(pair, ps.m, Log2<A>::v, Log2<kTileThreads / 64>::v);
+            const std::vector<PhaseLayout> L = plan_phase_layouts(pair, ps.m, Log2<A>::v, Log2<C::RegThreads / 64>::v);
             for (const PhaseLayout &P : L) {
                 SweepPhase h;
                 std::memset(&h, 0, sizeof(h));
@@ -1371,6 +1408,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
                 hphases.push_back(h);
             }
             ps.n_phases = (int)L.size();
+            QG_REQUIRE(ps.n_phases > 0, "adjoint sweep: register pass without phases");
         }
         pass_regs.push_back(use_regs);
         max_slots = std::max(max_slots, slots);
@@ -1401,14 +1439,15 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     for (size_t pi = 0; pi < passes.size() && e == hipSuccess; ++pi) {
         const SweepPass &ps = passes[pi];
         const u64 n_tiles = u64(1) << (n_lanes - ps.m);
-        const u64 piece = launch_piece_items() / kTileThreads;
+        const int threads = pass_regs[pi] ? C::RegThreads : kTileThreads;
+        const u64 piece = launch_piece_items() / (u64)threads;
         for (u64 first = 0; first < n_tiles; first += piece) {
             const u64 count = std::min(piece, n_tiles - first);
             if (pass_regs[pi])
-                hipLaunchKernelGGL((k_tile_sweep_reg<R, C::MaxBits, kTileThreads>), (unsigned)count, kTileThreads, 0, s,
-                                   phi, lam, ps, d_ops, d_ph, d_part, n_tiles, first);
+                hipLaunchKernelGGL((k_tile_sweep_reg<R, C::RegBits, C::RegThreads>), (unsigned)count,
+                                   C::RegThreads, 0, s, phi, lam, ps, d_ops, d_ph, d_part, n_tiles, first);
             else
-                hipLaunchKernelGGL((k_tile_sweep<R, C::MaxBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
+                hipLaunchKernelGGL((k_tile_sweep<R, C::LdsBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
                                    d_ops, d_part, n_tiles, first);
         }
         if (ps.n_slots > 0)   // terms of this pass's slots land at d_terms[gate index]
