@@ -997,23 +997,62 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
         // leave w_j = Im(conj(lambda_j) phi_j) unchanged (|factor| = 1), so the generator-Z
         // terms of a run of diagonal gates all use the same w: an RZ on a lane, wave or outer
         // bit contributes +-U with U = sum_j w_j, computed once per run (have_u).
+        // Runs of gates that leave the registers alone (fused diagonals, gates whose controls
+        // outside the tile are not all set here) go through an inner loop that carries only
+        // tf, U and the slot terms: the register arrays cross a loop back-edge (where the
+        // compiler inserts copies of them) once per register-changing gate, not once per gate.
         Cx<R> tf = cx<R>(R(1), R(0));
         bool pending = false, have_u = false;
         R u_sum = R(0);
-        for (int k = 0; k < P.n_ops; ++k) {
+        int k = 0;
+        while (k < P.n_ops) {
+            for (; k < P.n_ops; ++k) {
+                const SweepOp<R> &op = ops[pass.op_begin + P.op_begin + k];
+                const bool on = (base & op.gctrl) == op.gctrl;
+                const int kind = op.rkind, creg = op.creg, form = op.form;
+                if (on && (kind < kRegDiagThread || creg != 0)) break;   // changes the registers
+                R acc = R(0);
+                if (on) {
+                    const unsigned cthr = op.cthr;
+                    const bool ok = (off & cthr) == cthr;
+                    const u64 src = kind == kRegDiagThread ? (u64)off : base;
+                    const bool one = ((src >> op.rtarget) & 1) != 0;
+                    Cx<R> f = csel(one, op.udag.m11, op.udag.m00);
+                    f = csel(cthr == 0 || ok, f, cx<R>(R(1), R(0)));
+                    if ((form & 3) == 3) {
+                        if (!have_u) {
+                            u_sum = R(0);
+                            static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                                constexpr int j = decltype(jc)::value;
+                                u_sum += im_conj_mul(vl[j], vp[j]);
+                            });
+                            have_u = true;
+                        }
+                        acc = ok ? (one ? -u_sum : u_sum) : R(0);
+                    }
+                    tf = cmul(tf, f);
+                    pending = true;
+                }
+                if (op.slot >= 0) {
+                    const double s = wave_sum_lane63((double)acc);
+                    if (lane == 63) red[op.slot][wave] = s;
+                }
+            }
+            if (k == P.n_ops) break;
+            // one gate that changes the registers (it is on)
             const SweepOp<R> &op = ops[pass.op_begin + P.op_begin + k];
-            const bool on = (base & op.gctrl) == op.gctrl;
+            ++k;
             const unsigned cthr = op.cthr;
             const int creg = op.creg, tgt = op.rtarget, kind = op.rkind, form = op.form;
             const bool ok = (off & cthr) == cthr;
             const bool plain = cthr == 0 && creg == 0;
             R acc = R(0);
-            if (on && kind <= kRegPairReal) {
+            if (kind <= kRegPairReal) {
                 // tf (pending diagonal factor) commutes with the pair update, and the pair's
                 // gradient term is unchanged by a common unit-modulus factor of phi and lambda
                 reg_sweep_pair_at<R, A>(vp, vl, op.udag, creg, ok, plain, tgt, form, acc);
                 have_u = false;
-            } else if (on) {
+            } else {
                 const bool zgen = (form & 3) == 3;
                 const Cx<R> f0 = op.udag.m00, f1 = op.udag.m11;
                 if (kind <= kRegDiagPhase) {
@@ -1036,40 +1075,24 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
                             vl[j] = csel(c, ny, y);
                         }
                     });
-                } else {
+                } else {   // lane, wave or outer target with controls on register bits
                     const u64 src = kind == kRegDiagThread ? (u64)off : base;
                     const bool one = ((src >> tgt) & 1) != 0;
                     Cx<R> f = csel(one, f1, f0);
                     f = csel(cthr == 0 || ok, f, cx<R>(R(1), R(0)));
-                    if (creg == 0) {
+                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const bool c = (j & creg) == creg;   // scalar
+                        const Cx<R> x = vp[j], y = vl[j];
                         if (zgen) {
-                            if (!have_u) {
-                                u_sum = R(0);
-                                static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                                    constexpr int j = decltype(jc)::value;
-                                    u_sum += im_conj_mul(vl[j], vp[j]);
-                                });
-                                have_u = true;
-                            }
-                            acc = ok ? (one ? -u_sum : u_sum) : R(0);
+                            const R w = im_conj_mul(y, x);
+                            acc += (c && ok) ? (one ? -w : w) : R(0);
                         }
-                        tf = cmul(tf, f);
-                        pending = true;
-                    } else {
-                        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                            constexpr int j = decltype(jc)::value;
-                            const bool c = (j & creg) == creg;   // scalar
-                            const Cx<R> x = vp[j], y = vl[j];
-                            if (zgen) {
-                                const R w = im_conj_mul(y, x);
-                                acc += (c && ok) ? (one ? -w : w) : R(0);
-                            }
-                            if (c) {
-                                vp[j] = cmul(f, x);
-                                vl[j] = cmul(f, y);
-                            }
-                        });
-                    }
+                        if (c) {
+                            vp[j] = cmul(f, x);
+                            vl[j] = cmul(f, y);
+                        }
+                    });
                 }
             }
             if (op.slot >= 0) {
