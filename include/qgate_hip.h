@@ -234,10 +234,11 @@ int qghip_circuit_state(qghip_circuit *c, void *out, int64_t array_size);
 
 /* Many circuits of uneven qubit counts, each with its own parameters and Pauli-sum
  * coefficients, evaluated together: <H> and the full adjoint gradient of every item in one
- * captured HIP graph of persistent tile-pass kernels. Items added with the same circuit
- * handle, qubit count and observable terms (masks; the coefficients may differ) share a
- * group. The first run tunes tile size, workgroup size and grid size per group by timing
- * candidates, then captures the graph; later runs replay it with the current parameters.
+ * captured HIP graph of tile-pass kernels. Items added with the same circuit handle, qubit
+ * count and observable terms (masks; the coefficients may differ) share a group. The first
+ * run tunes per group the tile shape (register-resident or LDS passes, tile size) and the
+ * workgroup and grid size of the LDS passes by timing candidates, then captures the graph;
+ * later runs replay it with the current parameters.
  * Callers: variational workloads that evaluate many small circuits at once. */
 typedef struct qghip_batch qghip_batch;
 int qghip_batch_new(int prec, qghip_batch **out);
@@ -248,9 +249,12 @@ int qghip_batch_add(qghip_batch *b, const qghip_circuit *c, const double *params
                     const double *coefs, const uint64_t *xmask, const uint64_t *zmask, int *item);
 /* Replace the parameters of one item (used by the next run). */
 int qghip_batch_set_params(qghip_batch *b, int item, const double *params, int n_params);
-/* Fix tuning knobs instead of searching them (0: search): tile bits (1..12), threads per
- * workgroup (128 or 256), persistent workgroups per compute unit. Re-tunes on the next run. */
-int qghip_batch_set_tuning(qghip_batch *b, int tile_bits, int threads, int grid_per_cu);
+/* Fix tuning knobs instead of searching them (0, or -1 for registers: search): tile bits
+ * (1..12), threads per workgroup of the LDS passes (128 or 256), their persistent workgroups
+ * per compute unit, and registers (1: forward and sweep passes keep their tiles in registers
+ * where the group's qubit count allows full-size tiles, 0: LDS passes only). Re-tunes on the
+ * next run. */
+int qghip_batch_set_tuning(qghip_batch *b, int tile_bits, int threads, int grid_per_cu, int registers);
 /* Evaluate every item: values[i] = <H>_i; grads (if not null) receives each item's gradient
  * (its circuit's n_params entries) in item order. use_graph = 0 launches the same kernels
  * without the graph. */

@@ -8,9 +8,10 @@ of uneven qubit counts in one replayed HIP graph (csrc/k_batch.hip).
 
 Circuits built from the same ParametricCircuit object with observables of the same terms
 share a group: their state vectors are stacked into one array and every tile-pass kernel
-covers the whole group. The first run times tile size, workgroup size and grid size per
-group and keeps the fastest (`tuning` shows the choice), then captures the launch sequence
-of all groups into a graph that later runs replay.
+covers the whole group. The first run times the tile shape (register-resident or LDS passes,
+tile size) and the LDS passes' workgroup and grid size per group and keeps the fastest
+(`tuning` shows the choice), then captures the launch sequence of all groups into a graph
+that later runs replay.
 """
 import ctypes
 import json
@@ -35,7 +36,7 @@ def _lib():
             'qghip_batch_delete': [H],
             'qghip_batch_add': [H, H, P_D, ctypes.c_int, ctypes.c_int, P_D, P_U, P_U, ctypes.POINTER(ctypes.c_int)],
             'qghip_batch_set_params': [H, ctypes.c_int, P_D, ctypes.c_int],
-            'qghip_batch_set_tuning': [H, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+            'qghip_batch_set_tuning': [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
             'qghip_batch_run': [H, ctypes.c_int, P_D, P_D, ctypes.c_int64],
             'qghip_batch_describe': [H, ctypes.c_char_p, ctypes.c_int],
         }
@@ -87,9 +88,12 @@ class AdjointBatch:
         p = np.ascontiguousarray(np.asarray(params, np.float64).reshape(-1))
         hipruntime.check(_lib().qghip_batch_set_params(self.ptr, int(item), adjoint._dptr(p), len(p)))
 
-    def set_tuning(self, tile_bits=0, threads=0, grid_per_cu=0):
-        """Fix tuning knobs (0: let the autotuner search them)."""
-        hipruntime.check(_lib().qghip_batch_set_tuning(self.ptr, tile_bits, threads, grid_per_cu))
+    def set_tuning(self, tile_bits=0, threads=0, grid_per_cu=0, registers=-1):
+        """Fix tuning knobs (0, or -1 for registers: let the autotuner search them).
+
+        registers=1 keeps forward and sweep tiles in registers where a group's qubit count
+        allows full-size tiles; 0 uses LDS passes only."""
+        hipruntime.check(_lib().qghip_batch_set_tuning(self.ptr, tile_bits, threads, grid_per_cu, registers))
 
     def run(self, use_graph=True):
         """(values, grads): <H> per item and each item's gradient."""

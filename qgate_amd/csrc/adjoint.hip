@@ -35,28 +35,6 @@ inline unsigned blocks_for(u64 n, int b) { return (unsigned)((n + b - 1) / b); }
 
 typedef CircuitOp Op;
 
-GateMatrix rotation(int pauli, double theta) {
-    GateMatrix m;
-    const double c = std::cos(theta / 2.), s = std::sin(theta / 2.);
-    const std::complex<double> J(0., 1.);
-    if (pauli == 1) {          // RX
-        m.m[0][0] = c; m.m[0][1] = -J * s; m.m[1][0] = -J * s; m.m[1][1] = c;
-    } else if (pauli == 2) {   // RY
-        m.m[0][0] = c; m.m[0][1] = -s; m.m[1][0] = s; m.m[1][1] = c;
-    } else {                   // RZ
-        m.m[0][0] = std::exp(-J * (theta / 2.)); m.m[0][1] = 0.; m.m[1][0] = 0.;
-        m.m[1][1] = std::exp(J * (theta / 2.));
-    }
-    return m;
-}
-
-GateMatrix dagger(const GateMatrix &g) {
-    GateMatrix d;
-    for (int r = 0; r < 2; ++r)
-        for (int c = 0; c < 2; ++c) d.m[r][c] = std::conj(g.m[c][r]);
-    return d;
-}
-
 // Row-major (re, im) entries of the Pauli matrix X (1), Y (2) or Z (3); zeros otherwise.
 void set_generator(double *gen, int pauli) {
     for (int i = 0; i < 8; ++i) gen[i] = 0.;

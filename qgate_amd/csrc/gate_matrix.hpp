@@ -19,6 +19,10 @@ GateMatrix make_gate_matrix(int id, const double *args, int n_args, bool adjoint
 // Row-major (m00, m01, m10, m11) as interleaved (re, im) pairs.
 GateMatrix gate_matrix_from_array(const double *mat8);
 void gate_matrix_to_array(const GateMatrix &m, double *mat8);
+// exp(-i theta G / 2) for G = X (1), Y (2), Z (3): RX, RY, RZ.
+GateMatrix rotation(int pauli, double theta);
+// Conjugate transpose.
+GateMatrix dagger(const GateMatrix &g);
 
 // Rounded to the state precision, as the reference does before applying a gate
 // (qgate/simulator/src/CPUQubitProcessor.cpp:312, DeviceTypes.h:105-112).

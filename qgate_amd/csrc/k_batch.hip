@@ -32,6 +32,7 @@
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
 #include "tile_common.hpp"
+#include "tile_reg.hpp"
 
 using namespace qg;
 
@@ -229,12 +230,13 @@ __global__ __launch_bounds__(T) void k_batch_pauli(const Cx<R> *__restrict__ psi
 }
 
 // Backward sweep pass over psi and lambda tiles: per gate, the gradient term of a rotation
-// (partial[((item * n_slots) + slot) * tiles_per_item + tile]) and U^dagger on both.
+// (partial[slot * n_tiles + w] for tile w = item * tiles_per_item + tile index in the item)
+// and U^dagger on both.
 template <class R, int MB, int T>
 __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R> *__restrict__ lam, const BPass ps,
                                                    const BOp *__restrict__ ops, const BMats<R> *__restrict__ mats,
                                                    int n_gates, int n, double *__restrict__ partial,
-                                                   u64 tiles_per_item, u64 n_tiles) {
+                                                   u64 n_tiles) {
     __shared__ Cx<R> tp[1 << MB];
     __shared__ Cx<R> tl[1 << MB];
     __shared__ double scratch[T / 64];
@@ -242,7 +244,6 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
     for (u64 w = blockIdx.x; w < n_tiles; w += gridDim.x) {
         const u64 base = tile_base(w, ps.pos, ps.m);
         const u64 item = base >> n;
-        const u64 tin = w % tiles_per_item;
         load_tile<R, T, MB>(tp, phi, base, ps);
         load_tile<R, T, MB>(tl, lam, base, ps);
         __syncthreads();
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
             }
             if (op.slot >= 0) {
                 s = block_sum_tile<T>(s, scratch);
-                if (threadIdx.x == 0) partial[(item * (u64)ps.n_slots + (u64)op.slot) * tiles_per_item + tin] = s;
+                if (threadIdx.x == 0) partial[(u64)op.slot * n_tiles + w] = s;
             }
             __syncthreads();
         }
@@ -304,9 +305,10 @@ __global__ __launch_bounds__(T) void k_batch_sweep(Cx<R> *__restrict__ phi, Cx<R
     }
 }
 
-// out[(r / n_slots) * out_stride + slot_out[r % n_slots]] = sum of partial[r * len .. + len),
+// Row r = slot * n_items + item (slot-major: the tiles of all items for one slot are
+// consecutive): out[item * out_stride + slot_out[slot]] = sum of partial[r * len .. + len),
 // fixed order; one wavefront per row.
-__global__ __launch_bounds__(64) void k_rows_sum(const double *__restrict__ partial, u64 len, u64 n_rows, int n_slots,
+__global__ __launch_bounds__(64) void k_rows_sum(const double *__restrict__ partial, u64 len, u64 n_rows, u64 n_items,
                                                  const int *__restrict__ slot_out, int out_stride,
                                                  double *__restrict__ out) {
     const u64 r = blockIdx.x;
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(64) void k_rows_sum(const double *__restrict__ part
     for (u64 i = threadIdx.x; i < len; i += 64) s += row[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (threadIdx.x == 0) out[(r / (u64)n_slots) * (u64)out_stride + (u64)slot_out[r % (u64)n_slots]] = s;
+    if (threadIdx.x == 0) out[(r % n_items) * (u64)out_stride + (u64)slot_out[r / n_items]] = s;
 }
 
 // grads[item * n_params + p] = sum of terms[item * n_gates + g] over the rotations g with
@@ -349,6 +351,14 @@ struct BCfg<double> {
     static constexpr int SweepBits = 10;
     static constexpr int Low = 4;
 };
+// Register-resident passes: 256 threads, 2^12 / 2^11 amplitudes per forward tile (16 / 8 per
+// thread) and 2^11 / 2^10 per sweep tile (8 / 4 per thread and state), as in the
+// single-circuit passes.
+constexpr int kBatchRegThreads = 256;
+static_assert(BCfg<float>::MaxBits == TileCfg<float>::MaxBits && BCfg<double>::MaxBits == TileCfg<double>::MaxBits,
+              "forward tiles");
+static_assert(sizeof(BMats<float>) == 2 * sizeof(Mat2<float>) && sizeof(BMats<double>) == 2 * sizeof(Mat2<double>),
+              "BMats is (U, U^dagger) back to back");
 
 bool diag_matrix(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
 
@@ -358,15 +368,20 @@ struct Layout {
     int begin, end;
 };
 
-// Passes over consecutive entries: need[i] is the bit entry i requires in the tile (-1: none)
-// or, with several, need_masks[i].
-std::vector<Layout> plan_layouts(const std::vector<u64> &need_masks, int n, int mb, int low, bool c64) {
+// Passes over consecutive entries: need_masks[i] holds the bits entry i requires in the tile.
+// A pass also ends before it would hold more than max_ops entries, or more than max_counted
+// entries with counted[i] set (rotations of a sweep pass: one gradient slot each).
+std::vector<Layout> plan_layouts(const std::vector<u64> &need_masks, int n, int mb, int low, bool c64,
+                                 int max_ops = 1 << 30, const std::vector<char> *counted = nullptr,
+                                 int max_counted = 1 << 30) {
     std::vector<Layout> out;
     Layout cur;
+    int n_counted = 0;
     auto open = [&](int at) {
         cur.bits.clear();
         for (int b = 0; b < low; ++b) cur.bits.push_back(b);
         cur.begin = cur.end = at;
+        n_counted = 0;
     };
     auto missing = [&](u64 m) {
         std::vector<int> v;
@@ -378,7 +393,9 @@ std::vector<Layout> plan_layouts(const std::vector<u64> &need_masks, int n, int 
     for (int i = 0; i < (int)need_masks.size(); ++i) {
         QG_REQUIRE(__builtin_popcountll(need_masks[i] | (c64 ? 1 : 0)) <= mb, "gate or term does not fit a tile");
         std::vector<int> need = missing(need_masks[i]);
-        if ((int)(cur.bits.size() + need.size()) > mb) {
+        const int c = counted && (*counted)[i] ? 1 : 0;
+        if ((int)(cur.bits.size() + need.size()) > mb || cur.end - cur.begin == max_ops ||
+            n_counted + c > max_counted) {
             if (cur.end > cur.begin) out.push_back(cur);
             open(i);
             need = missing(need_masks[i]);
@@ -389,6 +406,7 @@ std::vector<Layout> plan_layouts(const std::vector<u64> &need_masks, int n, int 
         }
         for (int b : need) cur.bits.push_back(b);
         cur.end = i + 1;
+        n_counted += c;
     }
     if (cur.end > cur.begin || out.empty()) out.push_back(cur);
     for (Layout &l : out) {
@@ -419,8 +437,9 @@ BPass make_pass(const Layout &l, int local[64]) {
 
 struct Tuning {
     int tile_bits = 0;   // forward / Pauli tile bits (the sweep uses one fewer)
-    int threads = 256;
-    int grid_per_cu = 4;
+    int threads = 256;   // LDS passes
+    int grid_per_cu = 4; // LDS passes: persistent workgroups per compute unit
+    int registers = 0;   // 1: forward and sweep passes keep their tiles in registers
     double ms = 0.;      // measured time of one run of the group
 };
 
@@ -450,6 +469,14 @@ struct Group {
     int *d_slot_out = nullptr, *d_pptr = nullptr, *d_pgate = nullptr, *d_zero = nullptr;
     std::vector<BPass> fpasses, ppasses, spasses;
     std::vector<int> spass_slot_begin;      // offset of each sweep pass's slot table in d_slot_out
+    // register-resident passes (tune.registers): one program per forward pass (kernel
+    // arguments), sweep ops and phases in device tables
+    bool fregs = false, sregs = false;
+    std::vector<RegProgram<float>> rprog32;
+    std::vector<RegProgram<double>> rprog64;
+    std::vector<SweepPass> rspasses;
+    void *d_rsops = nullptr;                // SweepOp<R>[]
+    SweepPhase *d_rphases = nullptr;
     size_t n_items() const { return items.size(); }
 };
 
@@ -464,6 +491,7 @@ struct qghip_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int fixed_tile_bits = 0, fixed_threads = 0, fixed_grid = 0;   // > 0: skip tuning for that knob
+    int fixed_registers = -1;                                      // >= 0: skip tuning for that knob
 };
 
 namespace {
@@ -502,6 +530,48 @@ void drop_graph(qghip_batch *b) {
     b->graph = nullptr;
 }
 
+template <class R>
+std::vector<RegProgram<R>> &reg_programs(Group &g);
+template <>
+std::vector<RegProgram<float>> &reg_programs<float>(Group &g) {
+    return g.rprog32;
+}
+template <>
+std::vector<RegProgram<double>> &reg_programs<double>(Group &g) {
+    return g.rprog64;
+}
+
+// A gate's matrix as far as its form goes: fixed gates as they are, rotations at a generic
+// angle (every angle gives the same form: RX general / imaginary off-diagonal, RY real, RZ
+// diagonal with m00 != 1), so one classification holds for every item.
+GateMatrix form_matrix(const CircuitOp &op) { return op.kind == 1 ? rotation(op.pauli, 1.) : op.mat; }
+
+// Register-pass input of one forward gate (the fields plan_phases reads).
+template <class R>
+struct BRegInput {
+    Mat2<R> m;
+    u64 gctrl;
+    unsigned lctrl;
+    int target, kind;
+};
+
+// Tile-pass kind of a forward gate in a register pass: pair kinds split into swap / real /
+// general, local diagonals with m00 == 1 into kOpDiagPhase (from the matrix rounded to R).
+template <class R>
+int reg_forward_kind(const CircuitOp &op, int kind) {
+    const Mat2<R> mm = round_matrix<R>(form_matrix(op));
+    const R zero = R(0), one = R(1);
+    if (kind == kOpPair) {
+        const bool swap = op.kind == 0 && mm.m00.re == zero && mm.m00.im == zero && mm.m11.re == zero &&
+                          mm.m11.im == zero && mm.m01.re == one && mm.m01.im == zero && mm.m10.re == one &&
+                          mm.m10.im == zero;
+        const bool real = mm.m00.im == zero && mm.m01.im == zero && mm.m10.im == zero && mm.m11.im == zero;
+        return swap ? kOpPairSwap : real ? kOpPairReal : kOpPair;
+    }
+    if (kind == kOpDiagLocal && op.kind == 0 && mm.m00.re == one && mm.m00.im == zero) return kOpDiagPhase;
+    return kind;
+}
+
 // Plans and device tables of a group for a tuning (allocates; not during capture).
 template <class R>
 void build_group(Group &g, const Tuning &t) {
@@ -515,6 +585,10 @@ void build_group(Group &g, const Tuning &t) {
     const int mf = std::max(min_bits, std::min(t.tile_bits, n));
     const int ms = std::max(min_bits, std::min(t.tile_bits - 1, n));
     const int lowf = std::min(C::Low, mf), lows = std::min(C::Low, ms);
+    // register-resident passes need full-size tiles (the kernels' compile-time tile bits)
+    typedef TileCfg<R> TC;
+    const bool fregs = t.registers != 0 && mf == TC::MaxBits && use_register_tiles();
+    const bool sregs = t.registers != 0 && ms == C::SweepBits && use_register_tiles();
 
     // forward passes
     std::vector<u64> need(n_gates);
@@ -523,9 +597,11 @@ void build_group(Group &g, const Tuning &t) {
         const bool diag = op.kind == 1 ? op.pauli == 3 : diag_matrix(op.mat);
         need[i] = diag ? 0 : (u64(1) << op.target);
     }
-    std::vector<Layout> fl = plan_layouts(need, n, mf, lowf, c64);
+    std::vector<Layout> fl = plan_layouts(need, n, mf, lowf, c64, fregs ? TC::MaxOps : 1 << 30);
     std::vector<BOp> fops(std::max(1, n_gates));
     g.fpasses.clear();
+    std::vector<RegProgram<R>> &progs = reg_programs<R>(g);
+    progs.clear();
     for (const Layout &l : fl) {
         int local[64];
         BPass p = make_pass(l, local);
@@ -548,14 +624,41 @@ void build_group(Group &g, const Tuning &t) {
             o.slot = -1;
         }
         g.fpasses.push_back(p);
+        if (fregs) {
+            RegProgram<R> rp;
+            std::memset(&rp, 0, sizeof(rp));
+            rp.n_ops = p.n_ops;
+            rp.low = p.low;
+            std::memcpy(rp.pos, p.pos, sizeof(rp.pos));
+            std::vector<BRegInput<R>> in(std::max(1, p.n_ops));
+            for (int k = 0; k < p.n_ops; ++k) {
+                const BOp &o = fops[l.begin + k];
+                std::memset(&in[k], 0, sizeof(in[k]));
+                in[k].gctrl = o.gctrl;
+                in[k].lctrl = o.lctrl;
+                in[k].target = o.target;
+                in[k].kind = reg_forward_kind<R>(g.ops[l.begin + k], o.kind);
+            }
+            constexpr int A = (1 << TC::MaxBits) / kBatchRegThreads;
+            QG_REQUIRE(plan_phases<R>(rp, in.data(), p.m, Log2<A>::v, Log2<kBatchRegThreads / 64>::v),
+                       "batch: register pass with too many phases");
+            for (int k = 0; k < p.n_ops; ++k) rp.ops[k].gate = l.begin + k;
+            progs.push_back(rp);
+        }
     }
     // sweep passes: the gates in reverse order
     std::vector<u64> sneed(n_gates);
     for (int i = 0; i < n_gates; ++i) sneed[i] = need[n_gates - 1 - i];
-    std::vector<Layout> sl = plan_layouts(sneed, n, ms, lows, c64);
+    std::vector<char> srot(n_gates);
+    for (int i = 0; i < n_gates; ++i) srot[i] = g.ops[n_gates - 1 - i].kind == 1 ? 1 : 0;
+    std::vector<Layout> sl = plan_layouts(sneed, n, ms, lows, c64, 1 << 30, &srot, kMaxSweepSlots);
     std::vector<BOp> sops(std::max(1, n_gates));
+    std::vector<SweepOp<R>> rsops(std::max(1, n_gates));
+    std::memset(rsops.data(), 0, sizeof(SweepOp<R>) * rsops.size());
+    std::vector<SweepPhase> rphases;
     std::vector<int> slot_out;
     g.spasses.clear();
+    g.rspasses.clear();
     g.spass_slot_begin.clear();
     int max_slots = 1;
     for (const Layout &l : sl) {
@@ -582,11 +685,51 @@ void build_group(Group &g, const Tuning &t) {
             o.pauli = op.kind == 1 ? op.pauli : 0;
             o.slot = op.kind == 1 ? slots++ : -1;
             if (op.kind == 1) slot_out.push_back(gi);
+            SweepOp<R> &r = rsops[i];
+            r.gctrl = o.gctrl;
+            r.lctrl = o.lctrl;
+            r.target = o.target;
+            r.kind = o.kind;
+            r.slot = o.slot;
+            r.form = sweep_form(dagger(form_matrix(op)), o.pauli, o.kind != kOpPair);
+            r.gate = gi;
         }
         p.n_slots = slots;
         max_slots = std::max(max_slots, slots);
         g.spasses.push_back(p);
+        if (sregs) {
+            SweepPass rp;
+            std::memset(&rp, 0, sizeof(rp));
+            rp.m = p.m;
+            rp.low = p.low;
+            rp.op_begin = l.begin;
+            rp.n_ops = p.n_ops;
+            rp.n_slots = slots;
+            rp.phase_begin = (int)rphases.size();
+            std::memcpy(rp.pos, p.pos, sizeof(rp.pos));
+            std::vector<int> pair(p.n_ops);
+            for (int k = 0; k < p.n_ops; ++k)
+                pair[k] = rsops[l.begin + k].kind == kOpPair ? rsops[l.begin + k].target : -1;
+            constexpr int A = (1 << C::SweepBits) / kBatchRegThreads;
+            const std::vector<PhaseLayout> L =
+                plan_phase_layouts(pair, p.m, Log2<A>::v, Log2<kBatchRegThreads / 64>::v);
+            for (const PhaseLayout &P : L) {
+                for (int k = P.begin; k < P.end; ++k) {
+                    SweepOp<R> &r = rsops[l.begin + k];
+                    const RegSplit sp = split_for_phase(P, r.kind, r.target, r.lctrl);
+                    r.rkind = sp.kind;
+                    r.rtarget = sp.target;
+                    r.creg = sp.creg;
+                    r.cthr = sp.cthr;
+                }
+                rphases.push_back(phase_bits(P));
+            }
+            rp.n_phases = (int)L.size();
+            QG_REQUIRE(rp.n_ops == 0 || rp.n_phases > 0, "batch: register sweep pass without phases");
+            g.rspasses.push_back(rp);
+        }
     }
+    if (rphases.empty()) rphases.push_back(SweepPhase());
     if (slot_out.empty()) slot_out.push_back(0);
     // Pauli passes
     std::vector<u64> pneed;
@@ -660,6 +803,8 @@ void build_group(Group &g, const Tuning &t) {
     const size_t o_gates = carve(sizeof(BGate) * gates.size());
     const size_t o_fops = carve(sizeof(BOp) * fops.size());
     const size_t o_sops = carve(sizeof(BOp) * sops.size());
+    const size_t o_rsops = carve(sizeof(SweepOp<R>) * rsops.size());
+    const size_t o_rph = carve(sizeof(SweepPhase) * rphases.size());
     const size_t o_pg = carve(sizeof(PauliGroup) * std::max<size_t>(1, pgroups.size()));
     const size_t o_tm = carve(sizeof(BTermMask) * tmask.size());
     const size_t o_slot = carve(sizeof(int) * slot_out.size());
@@ -678,6 +823,8 @@ void build_group(Group &g, const Tuning &t) {
     g.d_gates = reinterpret_cast<BGate *>(g.block + o_gates);
     g.d_fops = reinterpret_cast<BOp *>(g.block + o_fops);
     g.d_sops = reinterpret_cast<BOp *>(g.block + o_sops);
+    g.d_rsops = g.block + o_rsops;
+    g.d_rphases = reinterpret_cast<SweepPhase *>(g.block + o_rph);
     g.d_pgroups = reinterpret_cast<PauliGroup *>(g.block + o_pg);
     g.d_tmask = reinterpret_cast<BTermMask *>(g.block + o_tm);
     g.d_slot_out = reinterpret_cast<int *>(g.block + o_slot);
@@ -689,6 +836,9 @@ void build_group(Group &g, const Tuning &t) {
     QG_HIP(hipMemcpyAsync(g.d_gates, gates.data(), sizeof(BGate) * gates.size(), hipMemcpyHostToDevice, s));
     QG_HIP(hipMemcpyAsync(g.d_fops, fops.data(), sizeof(BOp) * fops.size(), hipMemcpyHostToDevice, s));
     QG_HIP(hipMemcpyAsync(g.d_sops, sops.data(), sizeof(BOp) * sops.size(), hipMemcpyHostToDevice, s));
+    QG_HIP(hipMemcpyAsync(g.d_rsops, rsops.data(), sizeof(SweepOp<R>) * rsops.size(), hipMemcpyHostToDevice, s));
+    QG_HIP(hipMemcpyAsync(g.d_rphases, rphases.data(), sizeof(SweepPhase) * rphases.size(), hipMemcpyHostToDevice,
+                          s));
     if (!pgroups.empty())
         QG_HIP(hipMemcpyAsync(g.d_pgroups, pgroups.data(), sizeof(PauliGroup) * pgroups.size(), hipMemcpyHostToDevice, s));
     QG_HIP(hipMemcpyAsync(g.d_tmask, tmask.data(), sizeof(BTermMask) * tmask.size(), hipMemcpyHostToDevice, s));
@@ -698,6 +848,8 @@ void build_group(Group &g, const Tuning &t) {
     QG_HIP(hipMemcpyAsync(g.d_zero, &zero, sizeof(int), hipMemcpyHostToDevice, s));
     QG_HIP(hipMemcpyAsync(g.d_coefs, g.coefs.data(), sizeof(double) * g.coefs.size(), hipMemcpyHostToDevice, s));
     QG_HIP(hipStreamSynchronize(s));   // the tables above are host temporaries
+    g.fregs = fregs;
+    g.sregs = sregs;
     g.tune = t;
 }
 
@@ -719,9 +871,20 @@ void launch_group_t(Group &g) {
     const u64 n_units = (ni << n) / W;
     const int unit_bits = n - (W == 2 ? 1 : 0);
     hipLaunchKernelGGL((k_batch_zero<R>), (unsigned)((n_units + 255) / 256), 256, 0, s, psi, n_units, unit_bits);
-    for (const BPass &p : g.fpasses) {
+    const Mat2<R> *umats = reinterpret_cast<const Mat2<R> *>(mats);   // U of (item, gate): umats[2 (item n_gates + gate)]
+    const u64 piece = launch_piece_items() / kBatchRegThreads;         // register passes: one tile per workgroup
+    for (size_t k = 0; k < g.fpasses.size(); ++k) {
+        const BPass &p = g.fpasses[k];
         if (p.n_ops == 0) continue;
         const u64 n_tiles = ni << (n - p.m);
+        if (g.fregs) {
+            const RegProgram<R> &rp = reg_programs<R>(g)[k];
+            for (u64 first = 0; first < n_tiles; first += piece)
+                hipLaunchKernelGGL((k_tile_reg<R, TileCfg<R>::MaxBits, kBatchRegThreads, true>),
+                                   (unsigned)std::min(piece, n_tiles - first), kBatchRegThreads, 0, s, psi, rp, first,
+                                   umats, n_gates, n);
+            continue;
+        }
         const unsigned grid = (unsigned)std::min<u64>(n_tiles, grid_cap);
         hipLaunchKernelGGL((k_batch_pass<R, C::MaxBits, T>), grid, T, 0, s, psi, p, g.d_fops, mats, n_gates, n, n_tiles);
     }
@@ -734,20 +897,29 @@ void launch_group_t(Group &g) {
     {
         const BPass &p = g.ppasses.back();
         const u64 len = u64(1) << (n - p.m);
-        hipLaunchKernelGGL(k_rows_sum, (unsigned)ni, 64, 0, s, g.d_partial, len, ni, 1, g.d_zero, 1, g.d_values);
+        hipLaunchKernelGGL(k_rows_sum, (unsigned)ni, 64, 0, s, g.d_partial, len, ni, ni, g.d_zero, 1, g.d_values);
     }
     for (size_t k = 0; k < g.spasses.size(); ++k) {
         const BPass &p = g.spasses[k];
         if (p.n_ops == 0) continue;
         const u64 tpi = u64(1) << (n - p.m);
         const u64 n_tiles = ni * tpi;
-        const unsigned grid = (unsigned)std::min<u64>(n_tiles, grid_cap);
-        hipLaunchKernelGGL((k_batch_sweep<R, C::SweepBits, T>), grid, T, 0, s, psi, lam, p, g.d_sops, mats, n_gates, n,
-                           g.d_partial, tpi, n_tiles);
+        if (g.sregs) {
+            const SweepPass &rp = g.rspasses[k];
+            const Mat2<R> *udmats = umats + 1;   // U^dagger entries
+            for (u64 first = 0; first < n_tiles; first += piece)
+                hipLaunchKernelGGL((k_tile_sweep_reg<R, C::SweepBits, kBatchRegThreads, true>),
+                                   (unsigned)std::min(piece, n_tiles - first), kBatchRegThreads, 0, s, psi, lam, rp,
+                                   static_cast<const SweepOp<R> *>(g.d_rsops), g.d_rphases, g.d_partial, n_tiles,
+                                   first, udmats, n_gates, n);
+        } else {
+            const unsigned grid = (unsigned)std::min<u64>(n_tiles, grid_cap);
+            hipLaunchKernelGGL((k_batch_sweep<R, C::SweepBits, T>), grid, T, 0, s, psi, lam, p, g.d_sops, mats,
+                               n_gates, n, g.d_partial, n_tiles);
+        }
         if (p.n_slots > 0)
             hipLaunchKernelGGL(k_rows_sum, (unsigned)(ni * (u64)p.n_slots), 64, 0, s, g.d_partial, tpi,
-                               ni * (u64)p.n_slots, p.n_slots, g.d_slot_out + g.spass_slot_begin[k], n_gates,
-                               g.d_terms);
+                               ni * (u64)p.n_slots, ni, g.d_slot_out + g.spass_slot_begin[k], n_gates, g.d_terms);
     }
     if (g.n_params > 0) {
         const u64 nq = ni * (u64)g.n_params;
@@ -790,25 +962,38 @@ double time_group(Group &g, int reps) {
     return ms / reps;
 }
 
-// Autotuner: time each candidate (tile bits x workgroup size x persistent grid size) on the
-// group's own data and keep the fastest.
+// Autotuner: candidates are timed on the group's own structure and a sample of its items, in
+// two stages. First the tile shape: register-resident passes (full-size tiles; groups of at
+// least the sweep tile's qubit count) and LDS passes of up to three tile sizes, at 256
+// threads and 4 workgroups per compute unit. Then, for the fastest shape, the workgroup size
+// and persistent grid of the LDS passes (every group has Pauli passes in LDS). Knobs fixed
+// by qghip_batch_set_tuning are not searched.
 template <class R>
 void tune_group(qghip_batch *b, Group &g) {
     typedef BCfg<R> C;
-    std::vector<int> bits;
-    if (b->fixed_tile_bits > 0) {
-        bits.push_back(std::min(b->fixed_tile_bits, C::MaxBits));
-    } else {
-        for (int d = 0; d < 3; ++d) {
-            const int v = std::min(C::MaxBits - d, g.n);
-            if (v >= 1 && std::find(bits.begin(), bits.end(), v) == bits.end()) bits.push_back(v);
+    std::vector<std::pair<int, int>> shapes;   // (tile bits, registers)
+    auto add_shape = [&](int tb, int rg) {
+        if (std::find(shapes.begin(), shapes.end(), std::make_pair(tb, rg)) == shapes.end())
+            shapes.push_back(std::make_pair(tb, rg));
+    };
+    const bool reg_ok = use_register_tiles() && g.n >= C::SweepBits && b->fixed_registers != 0 &&
+                        (b->fixed_tile_bits == 0 || b->fixed_tile_bits == C::MaxBits);
+    if (reg_ok) add_shape(C::MaxBits, 1);
+    if (!reg_ok || b->fixed_registers != 1) {
+        if (b->fixed_tile_bits > 0) {
+            add_shape(std::min(b->fixed_tile_bits, C::MaxBits), 0);
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                const int v = std::min(C::MaxBits - d, g.n);
+                if (v >= 1) add_shape(v, 0);
+            }
         }
     }
-    std::vector<int> threads = b->fixed_threads > 0 ? std::vector<int>{b->fixed_threads} : std::vector<int>{128, 256};
-    std::vector<int> grids = b->fixed_grid > 0 ? std::vector<int>{b->fixed_grid} : std::vector<int>{2, 4, 8};
-    // time the candidates on a sample of the group's items (the structure is the same for
-    // every item, so the ranking carries over); the sample still spans several workgroups
-    // per compute unit
+    const std::vector<int> threads = b->fixed_threads > 0 ? std::vector<int>{b->fixed_threads} : std::vector<int>{256, 128};
+    const std::vector<int> grids = b->fixed_grid > 0 ? std::vector<int>{b->fixed_grid} : std::vector<int>{4, 8, 2};
+    // the sample: the structure is the same for every item, so the ranking carries over; it
+    // holds at most 2^24 amplitudes (and 64 items), which still spans several workgroups per
+    // compute unit
     Group sample;
     sample.n = g.n;
     sample.key = g.key;
@@ -817,24 +1002,29 @@ void tune_group(qghip_batch *b, Group &g) {
     sample.xmask = g.xmask;
     sample.zmask = g.zmask;
     sample.pgroups = g.pgroups;
-    const size_t keep = std::min<size_t>(g.n_items(), 64);
+    const size_t cap = std::max<size_t>(8, (size_t(1) << 24) >> std::min(g.n, 24));
+    const size_t keep = std::min<size_t>(g.n_items(), std::min<size_t>(64, cap));
     sample.items.assign(g.items.begin(), g.items.begin() + keep);
     sample.params.assign(g.params.begin(), g.params.begin() + keep * (size_t)g.n_params);
     sample.coefs.assign(g.coefs.begin(), g.coefs.begin() + keep * g.zmask.size() * 2);
     Tuning best;
     best.ms = -1.;
+    auto try_tuning = [&](int tb, int rg, int th, int gr) {
+        Tuning t;
+        t.tile_bits = tb;
+        t.registers = rg;
+        t.threads = th;
+        t.grid_per_cu = gr;
+        build_group<R>(sample, t);
+        t.ms = time_group<R>(sample, 2);
+        if (best.ms < 0. || t.ms < best.ms) best = t;
+    };
     try {
-        for (int tb : bits)
-            for (int th : threads)
-                for (int gr : grids) {
-                    Tuning t;
-                    t.tile_bits = tb;
-                    t.threads = th;
-                    t.grid_per_cu = gr;
-                    build_group<R>(sample, t);
-                    t.ms = time_group<R>(sample, 2);
-                    if (best.ms < 0. || t.ms < best.ms) best = t;
-                }
+        for (const auto &sh : shapes) try_tuning(sh.first, sh.second, threads[0], grids[0]);
+        const Tuning shape = best;
+        for (int th : threads)
+            for (int gr : grids)
+                if (th != threads[0] || gr != grids[0]) try_tuning(shape.tile_bits, shape.registers, th, gr);
     } catch (...) {
         free_group(sample);
         throw;
@@ -985,15 +1175,17 @@ int qghip_batch_set_params(qghip_batch *b, int item, const double *params, int n
     });
 }
 
-int qghip_batch_set_tuning(qghip_batch *b, int tile_bits, int threads, int grid_per_cu) {
+int qghip_batch_set_tuning(qghip_batch *b, int tile_bits, int threads, int grid_per_cu, int registers) {
     return guarded_batch([&] {
         require(b != nullptr, "null argument");
+        require(registers >= -1 && registers <= 1, "registers must be -1, 0 or 1");
         require(tile_bits == 0 || (1 <= tile_bits && tile_bits <= 12), "tile_bits out of range");
         require(threads == 0 || threads == 128 || threads == 256, "threads must be 128 or 256");
         require(grid_per_cu >= 0 && grid_per_cu <= 64, "grid_per_cu out of range");
         b->fixed_tile_bits = tile_bits;
         b->fixed_threads = threads;
         b->fixed_grid = grid_per_cu;
+        b->fixed_registers = registers;
         b->tuned = false;
         b->built = false;
     });
@@ -1053,12 +1245,15 @@ int qghip_batch_describe(const qghip_batch *b, char *buf, int size) {
         std::string out = "[";
         for (size_t k = 0; k < b->groups.size(); ++k) {
             const Group &g = b->groups[k];
-            char line[256];
+            char line[512];
             snprintf(line, sizeof(line),
                      "%s{\"qubits\": %d, \"items\": %zu, \"gates\": %zu, \"tile_bits\": %d, \"threads\": %d, "
-                     "\"grid_per_cu\": %d, \"tuned_ms\": %.4f, \"forward_passes\": %zu, \"sweep_passes\": %zu}",
+                     "\"grid_per_cu\": %d, \"registers\": %d, \"forward_in_registers\": %d, "
+                     "\"sweep_in_registers\": %d, \"tuned_ms\": %.4f, \"forward_passes\": %zu, "
+                     "\"sweep_passes\": %zu}",
                      k ? ", " : "", g.n, g.n_items(), g.ops.size(), g.tune.tile_bits, g.tune.threads,
-                     g.tune.grid_per_cu, g.tune.ms, g.fpasses.size(), g.spasses.size());
+                     g.tune.grid_per_cu, g.tune.registers, g.fregs ? 1 : 0, g.sregs ? 1 : 0, g.tune.ms,
+                     g.fpasses.size(), g.spasses.size());
             out += line;
         }
         out += "]";

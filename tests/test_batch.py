@@ -111,6 +111,75 @@ def test_mixed_circuits_and_observables(hip_module, dtype):
         assert np.max(np.abs(g - rg)) < GTOL[dtype]
 
 
+# full-size tile bits of the forward and sweep passes (complex128, complex64)
+FWD_BITS = {np.float64: 11, np.float32: 12}
+SWEEP_BITS = {np.float64: 10, np.float32: 11}
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('registers', [1, 0])
+def test_register_and_lds_passes(hip_module, dtype, registers):
+    """Register-resident passes (forced on, for groups with full-size tiles) and LDS passes
+    (forced) both give the single-circuit results. The circuits mix controlled rotations and
+    fixed gates, so controls fall on register, lane/wave and outer bits of the tiles."""
+    from qgate_amd.batch import AdjointBatch
+    batch = AdjointBatch(dtype)
+    cases = []
+    for seed, n in ((5, 10), (6, 11), (7, 12), (8, 14), (9, 16)):
+        spec = ac.mixed_spec(n, 120, seed=seed, n_params=20)
+        circ = spec.build(dtype)
+        h = ac.random_observable(n, 6, seed=seed + 30)
+        for k in range(2):
+            params = np.random.RandomState(seed * 7 + k).uniform(-math.pi, math.pi, spec.n_params)
+            batch.add(circ, params, h)
+            cases.append((circ, params, h))
+    batch.set_tuning(registers=registers)
+    values, grads = batch.run()
+    for (circ, params, h), v, g in zip(cases, values, grads):
+        rv, rg = circ.gradient(params, h)
+        assert abs(v - rv) < VTOL[dtype] * 10
+        assert np.max(np.abs(g - rg)) < GTOL[dtype]
+    for g in batch.tuning:
+        n = g['qubits']
+        assert g['forward_in_registers'] == int(registers == 1 and n >= FWD_BITS[dtype]), g
+        assert g['sweep_in_registers'] == int(registers == 1 and n >= SWEEP_BITS[dtype]), g
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
+def test_pass_caps(hip_module, dtype):
+    """Dense circuits on the low qubits would fit one tile: passes still end at the register
+    program's gate limit (forward) and at the gradient-slot limit (sweep)."""
+    from qgate_amd.batch import AdjointBatch
+    from qgate_amd.adjoint import ParametricCircuit
+    n = 12
+    circ = ParametricCircuit(n, dtype)
+    k = 0
+    for layer in range(40):
+        for q in range(5):
+            circ.ry(q, k)
+            circ.rz(q, k + 1)
+            k += 2
+        circ.cz(layer % 5, (layer + 1) % 5)
+    batch = AdjointBatch(dtype)
+    rs = np.random.RandomState(21)
+    h = ac.random_observable(n, 5, seed=22)
+    items = []
+    for _ in range(3):
+        params = rs.uniform(-math.pi, math.pi, k)
+        batch.add(circ, params, h)
+        items.append(params)
+    batch.set_tuning(registers=1)
+    values, grads = batch.run()
+    for params, v, g in zip(items, values, grads):
+        rv, rg = circ.gradient(params, h)
+        assert abs(v - rv) < VTOL[dtype] * 10
+        assert np.max(np.abs(g - rg)) < GTOL[dtype]
+    (grp,) = batch.tuning
+    assert grp['sweep_in_registers'] == 1
+    assert grp['sweep_passes'] >= math.ceil(k / 96)
+    assert grp['forward_passes'] >= math.ceil(len(range(40)) * 11 / (40 if dtype == np.float32 else 24))
+
+
 def test_batch_against_oracle_parameter_shift(hip_module, port_module):
     from qgate_amd.batch import AdjointBatch
     batch = AdjointBatch(np.float64)
@@ -139,3 +208,5 @@ def test_batch_argument_errors(hip_module):
         batch.add(c, np.zeros(3), PauliSum.zz_x(np.ones(3), np.ones(4)))      # too few parameters
     with pytest.raises(ValueError):
         batch.set_tuning(13, 0, 0)
+    with pytest.raises(ValueError):
+        batch.set_tuning(registers=2)
