@@ -298,6 +298,8 @@ __device__ __forceinline__ double row_sum(const double *__restrict__ row, u64 n,
 }
 
 constexpr int kRowThreads = 1024;
+// Device memory for the sweep's per-tile gradient partials of consecutive passes.
+constexpr size_t kSweepPartialBytes = size_t(256) << 20;
 
 // terms[slot_gate[r]] = sum over tiles of partial[r * n_tiles + tile], fixed order; one
 // workgroup per slot row.
@@ -489,7 +491,6 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     std::vector<SweepPhase> hphases;
     std::vector<int> slot_gate;
     std::vector<SweepPass> passes;
-    std::vector<int> pass_slot_begin;
     std::vector<bool> pass_regs;
     int max_slots = 0;
     for (const Plan &pl : plans) {
@@ -508,7 +509,6 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         int local[64];
         for (int b = 0; b < 64; ++b) local[b] = -1;
         for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
-        pass_slot_begin.push_back((int)slot_gate.size());
         int slots = 0;
         for (int i = pl.begin; i < pl.end; ++i) {
             const SweepGate &g = gates[i];
@@ -569,9 +569,17 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     if (slot_gate.empty()) slot_gate.push_back(0);
 
     hipStream_t s = stream();
-    size_t part_bytes = 8;
-    for (const SweepPass &ps : passes)
-        part_bytes = std::max(part_bytes, sizeof(double) * (size_t)max_slots * (size_t)(u64(1) << (n_lanes - ps.m)));
+    // Every pass has the same tile count (tiles are filled up to mb bits). The partial rows of
+    // consecutive passes accumulate in one buffer (up to kSweepPartialBytes) and are summed
+    // by one launch when it is full or the sweep ends, not by a small launch after each pass.
+    const u64 n_tiles = u64(1) << (n_lanes - passes[0].m);
+    for (const SweepPass &ps : passes) QG_REQUIRE(ps.m == passes[0].m, "adjoint sweep: passes of unequal tiles");
+    int total_slots = 0;
+    for (const SweepPass &ps : passes) total_slots += ps.n_slots;
+    const size_t row_bytes = sizeof(double) * (size_t)n_tiles;
+    const int cap_slots = (int)std::max<size_t>(
+        (size_t)max_slots, std::min<size_t>((size_t)total_slots, kSweepPartialBytes / row_bytes));
+    const size_t part_bytes = std::max<size_t>(8, row_bytes * (size_t)cap_slots);
     const size_t ops_bytes = sizeof(SweepOp<R>) * hops.size();
     const size_t ph_bytes = sizeof(SweepPhase) * hphases.size();
     const size_t sg_bytes = sizeof(int) * slot_gate.size();
@@ -586,24 +594,35 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     hipError_t e = hipMemcpyAsync(d_ops, hops.data(), ops_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ph, hphases.data(), ph_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_sg, slot_gate.data(), sg_bytes, hipMemcpyHostToDevice, s);
+    int pending = 0, pending_begin = 0;   // rows in the buffer, and the slot_gate index of the first
+    auto sum_rows = [&]() {                // terms of the pending rows land at d_terms[gate index]
+        if (pending > 0)
+            hipLaunchKernelGGL(k_slot_rows, (unsigned)pending, kRowThreads, 0, s, d_part, n_tiles,
+                               d_sg + pending_begin, d_terms);
+        pending_begin += pending;
+        pending = 0;
+    };
     for (size_t pi = 0; pi < passes.size() && e == hipSuccess; ++pi) {
         const SweepPass &ps = passes[pi];
-        const u64 n_tiles = u64(1) << (n_lanes - ps.m);
+        if (pending + ps.n_slots > cap_slots) sum_rows();
+        double *part = d_part + (u64)pending * n_tiles;
         const int threads = pass_regs[pi] ? C::RegThreads : kTileThreads;
         const u64 piece = launch_piece_items() / (u64)threads;
         for (u64 first = 0; first < n_tiles; first += piece) {
             const u64 count = std::min(piece, n_tiles - first);
             if (pass_regs[pi])
                 hipLaunchKernelGGL((k_tile_sweep_reg<R, C::RegBits, C::RegThreads, false>), (unsigned)count,
-                                   C::RegThreads, 0, s, phi, lam, ps, d_ops, d_ph, d_part, n_tiles, first,
+                                   C::RegThreads, 0, s, phi, lam, ps, d_ops, d_ph, part, n_tiles, first,
                                    (const Mat2<R> *)nullptr, 0, 0);
             else
                 hipLaunchKernelGGL((k_tile_sweep<R, C::LdsBits>), (unsigned)count, kTileThreads, 0, s, phi, lam, ps,
-                                   d_ops, d_part, n_tiles, first);
+                                   d_ops, part, n_tiles, first);
         }
-        if (ps.n_slots > 0)   // terms of this pass's slots land at d_terms[gate index]
-            hipLaunchKernelGGL(k_slot_rows, (unsigned)ps.n_slots, kRowThreads, 0, s, d_part, n_tiles, d_sg + pass_slot_begin[pi],
-                               d_terms);
+        pending += ps.n_slots;
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        sum_rows();
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);   // the tables are host temporaries

@@ -203,6 +203,34 @@ def test_hea24_gradient_by_shift_on_device(hip_module, dtype):
 
 
 @pytest.mark.gpu
+def test_gradient_rows_of_many_passes(hip_module):
+    """26 qubits (complex128: 2^16 tiles, 512 KiB of partial sums per rotation) with 1248
+    rotations: the per-tile partials outgrow their 256 MiB buffer and are summed in several
+    launches during the sweep. Seeded gradient entries against parameter-shift differences,
+    and every entry against the batched path, which sums the rows of each pass by itself."""
+    from qgate_amd.batch import AdjointBatch
+    n, layers = 26, 24
+    spec = ac.hea_spec(n, layers)
+    h = PauliSum.zz_x(np.random.RandomState(2601).normal(size=n - 1), np.random.RandomState(2602).normal(size=n))
+    params = np.random.RandomState(26).uniform(-math.pi, math.pi, spec.n_params)
+    circ = spec.build(np.float64)
+    value, grad = circ.gradient(params, h)
+    for k in np.random.RandomState(27).choice(spec.n_params, 6, replace=False):
+        pp, pm = params.copy(), params.copy()
+        pp[k] += math.pi / 2
+        pm[k] -= math.pi / 2
+        ref = (circ.expectation(pp, h) - circ.expectation(pm, h)) / 2.
+        assert abs(grad[k] - ref) < 1e-10, (k, grad[k], ref)
+    batch = AdjointBatch(np.float64)
+    batch.add(circ, params, h)
+    batch.set_tuning(11, 256, 4, 1)
+    values, grads = batch.run()
+    batch.delete()
+    assert abs(values[0] - value) < 1e-10
+    assert np.max(np.abs(grads[0] - grad)) < 1e-10
+
+
+@pytest.mark.gpu
 def test_torch_autograd(hip_module):
     import torch
     from qgate_amd.adjoint import Expectation
