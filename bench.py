@@ -40,7 +40,8 @@ cpu_baseline: the reference's own C++ CPU runtime (oracle/_ref, or the C restate
 that is not built) applying the first gates of the same circuit to a 30-qubit complex64
 state in host memory, for about 15 s of work, all host cores.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-large]
+    python bench.py --batch-only      # the batched adjoint alone, one JSON object
 """
 import argparse
 import json
@@ -245,7 +246,8 @@ def batch_timing(n_circuits=4096, layers=10, seed=5, loop_sample=512):
             'graph_ms': round(graph_s * 1e3, 2), 'first_run_with_tuning_ms': round(first_s * 1e3, 1),
             'loop_ms_scaled': round(loop_s * 1e3, 1), 'loop_sample': loop_sample,
             'speedup_vs_loop': round(loop_s / graph_s, 2), 'circuits_per_s': round(n_circuits / graph_s, 1),
-            'groups': [{k: g[k] for k in ('qubits', 'items', 'tile_bits', 'threads', 'grid_per_cu')} for g in tuning]}
+            'groups': [{k: g[k] for k in ('qubits', 'items', 'tile_bits', 'threads', 'grid_per_cu', 'registers',
+                                          'forward_in_registers', 'sweep_in_registers')} for g in tuning]}
 
 
 def pauli_timing(r, n, reps=5):
@@ -384,6 +386,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-batch', action='store_true')
     ap.add_argument('--no-large', action='store_true')
+    ap.add_argument('--batch-only', action='store_true',
+                    help='only the batched adjoint (tuning run, three graph replays, a short single-circuit '
+                         'loop); prints its JSON object (for profiler runs)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the per-target gate timing loop and a 2 GiB copy kernel (for counter runs)')
     args = ap.parse_args()
@@ -391,6 +396,10 @@ def main():
         print('bench.py runs on one GPU only: the state-vector backend does not distribute qubits '
               'across GPUs (got --gpus {}).'.format(args.gpus), file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        print(json.dumps(batch_timing(loop_sample=32)), flush=True)
+        return
 
     from qgate_amd import workloads
     from qgate_amd.simulator import hipruntime

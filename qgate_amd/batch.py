@@ -65,8 +65,9 @@ class AdjointBatch:
         self.delete()
 
     def delete(self):
-        if getattr(self, 'ptr', None) is not None and hipruntime._lib is not None:
-            hipruntime._lib.qghip_batch_delete(self.ptr)
+        rt = hipruntime   # None while the interpreter shuts down (the process frees the memory)
+        if getattr(self, 'ptr', None) is not None and rt is not None and rt._lib is not None:
+            rt._lib.qghip_batch_delete(self.ptr)
         self.ptr = None
 
     def __len__(self):
