@@ -14,7 +14,9 @@ uncontrolled 2x2 gate 2 S 2^n, c-controlled 2 S 2^(n-c), diagonal gates with m00
 roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, timed with
 HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
 launch time, against 8 TB/s; traffic = HBM bytes per launch measured with the FETCH_SIZE and
-WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md).
+WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md); copy_GBs = the HIP runtime's own
+device-to-device copy of 4 GiB (read + write bytes, best of 5), and frac_of_copy =
+achieved / copy_GBs.
 
 large_state_34q: a 34-qubit complex64 state (128 GiB): per-target gate GB/s split at target 7,
 P(0) throughput and a 30-of-34-qubit sampling-pool build (prefix sums over 2^34 amplitudes).
@@ -162,6 +164,52 @@ def roofline(runner, n, amp, reps=4):
             'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
             'by_kernel_GBs': split,
             'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
+
+
+def copy_rate_GBs(gib=4, reps=5):
+    """Measured device-to-device copy rate (the HIP runtime's own copy of a gib-GiB buffer,
+    hipMemcpyAsync on the null stream timed with HIP events, best of reps; bytes = read +
+    write): what a plain copy reaches on this HBM."""
+    import ctypes
+    hip = ctypes.CDLL('libamdhip64.so')
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemsetAsync.argtypes = [vp, ctypes.c_int, sz, vp]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError('HIP error {}'.format(rc))
+
+    nbytes = gib << 30
+    src, dst, e0, e1 = vp(), vp(), vp(), vp()
+    ok(hip.hipMalloc(ctypes.byref(src), nbytes))
+    ok(hip.hipMalloc(ctypes.byref(dst), nbytes))
+    ok(hip.hipEventCreate(ctypes.byref(e0)))
+    ok(hip.hipEventCreate(ctypes.byref(e1)))
+    try:
+        ok(hip.hipMemsetAsync(src, 1, nbytes, None))
+        best = None
+        for _ in range(reps + 1):   # the first copy warms up
+            ok(hip.hipEventRecord(e0, None))
+            ok(hip.hipMemcpyAsync(dst, src, nbytes, 3, None))   # hipMemcpyDeviceToDevice
+            ok(hip.hipEventRecord(e1, None))
+            ok(hip.hipEventSynchronize(e1))
+            ms = ctypes.c_float()
+            ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
+            best = ms.value if best is None else min(best, ms.value)
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
 
 
 def device_copy_ms(module, n):
@@ -485,7 +533,10 @@ def main():
 
     roof = roofline(r, n, amp)
     r.delete()
-    print('[bench] roofline: {:.1f} GB/s'.format(roof['achieved']), file=sys.stderr, flush=True)
+    roof['copy_GBs'] = copy_rate_GBs()
+    roof['frac_of_copy'] = round(roof['achieved'] / roof['copy_GBs'], 4)
+    print('[bench] roofline: {:.1f} GB/s, device copy {:.1f} GB/s'.format(roof['achieved'], roof['copy_GBs']),
+          file=sys.stderr, flush=True)
 
     adjoint = {}
     for name, dt in (('complex64', np.float32), ('complex128', np.float64)):

@@ -54,8 +54,20 @@ struct GateArgs {
     u64 n_items;
     u64 dist;           // partner distance in units (strided) or lanes (wave)
     u64 lane_ctrl;      // control bits of the unit index below the wavefront chunk (k_wave_pair)
+    u64 unit_chk;       // control bits of the unit index tested per work item (whole lines, below)
     unsigned amp_ctrl;  // control bits inside a unit (complex64 bit 0)
 };
+
+// Controls (and k_phase targets) on the lowest unit-index bits are not folded into the
+// work-item index: folded, they would leave touched runs shorter than a 128-byte line
+// (8 units), and the half-used lines, read and partly written, measured about twice the time
+// of a full pass at 30 qubits (CX with its control on amplitude bit 1 or 2: 5.2 ms against
+// 2.65 ms). Below kWholeLineBits such bits are tested per work item instead, over every unit,
+// and units that fail the test are written back unchanged, so every line is read and
+// written whole. k_wave_pair reads whole lines anyway; it writes them whole (lanes that
+// fail still store) for bits below kWaveWholeLineBits.
+constexpr int kWholeLineBits = 3;
+constexpr int kWaveWholeLineBits = 2;
 
 template <class R>
 __device__ __forceinline__ void mix(const Mat2<R> &m, Cx<R> &x0, Cx<R> &x1) {
@@ -78,9 +90,11 @@ __global__ __launch_bounds__(B) void k_strided_pair(Cx<R> *__restrict__ a, const
     const u64 u0 = scatter<NI>(w, g.ins) | g.item_set, u1 = u0 | g.dist;
     Amps<R> x0 = load_unit(a, u0);
     Amps<R> x1 = load_unit(a, u1);
+    if ((u0 & g.unit_chk) == g.unit_chk) {
 #pragma unroll
-    for (int j = 0; j < Unit<R>::W; ++j)
-        if (amp_on<R>(j, g.amp_ctrl)) mix(g.m, x0.v[j], x1.v[j]);
+        for (int j = 0; j < Unit<R>::W; ++j)
+            if (amp_on<R>(j, g.amp_ctrl)) mix(g.m, x0.v[j], x1.v[j]);
+    }
     store_unit(a, u0, x0);
     store_unit(a, u1, x1);
 }
@@ -101,11 +115,12 @@ __global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const G
 #pragma unroll
     for (int j = 0; j < Unit<R>::W; ++j) p.v[j] = shfl_xor_cx(x.v[j], (int)g.dist);
     if (((u64)lane & g.lane_ctrl) != g.lane_ctrl) return;
+    const bool on = ((u64)lane & g.unit_chk) == g.unit_chk;   // off: written back unchanged
     const bool hi = ((u64)lane & g.dist) != 0;
     Amps<R> y = x;
 #pragma unroll
     for (int j = 0; j < Unit<R>::W; ++j) {
-        if (!amp_on<R>(j, g.amp_ctrl)) continue;
+        if (!on || !amp_on<R>(j, g.amp_ctrl)) continue;
         Cx<R> lo_v = hi ? p.v[j] : x.v[j];
         Cx<R> hi_v = hi ? x.v[j] : p.v[j];
         mix(g.m, lo_v, hi_v);
@@ -121,7 +136,7 @@ __global__ __launch_bounds__(B) void k_unit_pair(Cx<float> *__restrict__ a, cons
     if (w >= g.n_items) return;
     const u64 u = scatter<NI>(w, g.ins) | g.item_set;
     Amps<float> x = load_unit(a, u);
-    mix(g.m, x.v[0], x.v[1]);
+    if ((u & g.unit_chk) == g.unit_chk) mix(g.m, x.v[0], x.v[1]);
     store_unit(a, u, x);
 }
 
@@ -132,9 +147,11 @@ __global__ __launch_bounds__(B) void k_phase(Cx<R> *__restrict__ a, const GateAr
     if (w >= g.n_items) return;
     const u64 u = scatter<NI>(w, g.ins) | g.item_set;
     Amps<R> x = load_unit(a, u);
+    if ((u & g.unit_chk) == g.unit_chk) {
 #pragma unroll
-    for (int j = 0; j < Unit<R>::W; ++j)
-        if (amp_on<R>(j, g.amp_ctrl)) x.v[j] = cmul(g.m.m11, x.v[j]);
+        for (int j = 0; j < Unit<R>::W; ++j)
+            if (amp_on<R>(j, g.amp_ctrl)) x.v[j] = cmul(g.m.m11, x.v[j]);
+    }
     store_unit(a, u, x);
 }
 
@@ -183,8 +200,20 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
     g.m = round_matrix<R>(gm);
     g.amp_ctrl = 0;
     g.lane_ctrl = 0;
+    g.unit_chk = 0;
     g.dist = 0;
     g.item_set = 0;
+    // positions of the unit index below kWholeLineBits: tested per work item, not folded
+    auto split_low = [&](std::vector<int> &pos) {
+        std::vector<int> folded;
+        for (int p : pos) {
+            if (p < kWholeLineBits)
+                g.unit_chk |= u64(1) << p;
+            else
+                folded.push_back(p);
+        }
+        pos.swap(folded);
+    };
     // controls inside a unit (complex64 bit 0)
     std::vector<int> unit_ctrl;                // control positions in the unit index
     for (int c : ctrl) {
@@ -203,6 +232,7 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
             g.amp_ctrl |= 1u << target;
         else
             pos.push_back(target - lw);
+        split_low(pos);
         g.ins = make_insert(pos);
         for (int p : pos) g.item_set |= u64(1) << p;
         g.n_items = u64(1) << (unit_bits - (int)pos.size());
@@ -218,6 +248,7 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
 
     if constexpr (std::is_same<R, float>::value) {
         if (target == 0) {                                     // pair inside the unit
+            split_low(unit_ctrl);
             g.ins = make_insert(unit_ctrl);
             for (int p : unit_ctrl) g.item_set |= u64(1) << p;
             g.n_items = u64(1) << (unit_bits - (int)unit_ctrl.size());
@@ -238,6 +269,8 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
         for (int p : unit_ctrl) {
             if (p >= 6)
                 chunk_ctrl.push_back(p - 6);
+            else if (p < kWaveWholeLineBits)
+                g.unit_chk |= u64(1) << p;
             else
                 g.lane_ctrl |= u64(1) << p;
         }
@@ -256,6 +289,7 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
         return;
     }
 
+    split_low(unit_ctrl);
     std::vector<int> pos(unit_ctrl);
     pos.push_back(tu);
     g.ins = make_insert(pos);

@@ -66,6 +66,36 @@ def test_every_gate_every_target(hip_module, port_module, dtype):
 
 
 @pytest.mark.parametrize('dtype', DTYPES)
+def test_controls_on_low_bits(hip_module, port_module, dtype):
+    """Every (control, target) pair of an 11-qubit state for a pair gate (U3, CX) and a phase
+    gate (cu1, CZ), plus two-control gates whose controls sit on the lowest bits: controls on
+    the lowest unit-index bits are tested per work item over whole lines, the others folded
+    into the work-item index, across the unit, wavefront and strided kernels."""
+    n = 11
+    rs = np.random.RandomState(12)
+    q, prep_ops = workloads.brickwork(n, 2, seed=13)
+    ops = list(prep_ops)
+    for c in range(n):
+        for t in range(n):
+            if c == t:
+                continue
+            ops.append(ctrl(q[c]).U3(*rs.uniform(0, 6.3, 3))(q[t]))
+            ops.append(ctrl(q[c]).X(q[t]))
+            ops.append(ctrl(q[c]).U1(rs.uniform(0, 6.3))(q[t]))
+            ops.append(ctrl(q[c]).Z(q[t]))
+    for c0 in range(4):
+        for c1 in range(c0 + 1, 5):
+            for t in range(n):
+                if t in (c0, c1):
+                    continue
+                ops.append(ctrl(q[c0], q[c1]).U3(*rs.uniform(0, 6.3, 3))(q[t]))
+                ops.append(ctrl(q[c0], q[c1]).S(q[t]))
+    a = _sim(hip_module, ops, 'one_static', dtype, q)
+    b = _sim(port_module, ops, 'one_static', dtype, q)
+    assert max_abs(a.qubits.states[:], b.qubits.states[:]) < tol(dtype) * 10
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
 @pytest.mark.parametrize('prep', ['dynamic', 'one_static'])
 def test_brickwork_16q(hip_module, port_module, dtype, prep):
     """16-qubit random brickwork of 20 layers (U3 on all qubits, CX on alternating pairs):
