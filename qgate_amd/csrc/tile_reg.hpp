@@ -134,6 +134,36 @@ __device__ __forceinline__ R im_conj_mul(const Cx<R> &a, const Cx<R> &b) {
     return a.re * b.im - a.im * b.re;
 }
 
+template <class R>
+__device__ __forceinline__ R fma_r(R a, R b, R c) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_fmaf(a, b, c);
+    else
+        return __builtin_fma(a, b, c);
+}
+
+// acc + Im(conj(a) b), as two FMAs into the accumulator
+template <class R>
+__device__ __forceinline__ R add_im_conj_mul(R acc, const Cx<R> &a, const Cx<R> &b) {
+    return fma_r(-a.im, b.re, fma_r(a.re, b.im, acc));
+}
+
+// acc + the gradient term of a pair (p0, p1) / (l0, l1) for generator G (1 X, 2 Y, 3 Z):
+// Im<l|G|p> over the pair, four FMAs into the accumulator
+template <class R, int G>
+__device__ __forceinline__ R add_pair_term(R acc, const Cx<R> &p0, const Cx<R> &p1, const Cx<R> &l0,
+                                           const Cx<R> &l1) {
+    if constexpr (G == 1) return add_im_conj_mul(add_im_conj_mul(acc, l0, p1), l1, p0);
+    if constexpr (G == 2) {
+        acc = fma_r(l1.re, p0.re, acc);
+        acc = fma_r(l1.im, p0.im, acc);
+        acc = fma_r(-l0.re, p1.re, acc);
+        return fma_r(-l0.im, p1.im, acc);
+    }
+    if constexpr (G == 3) return add_im_conj_mul(add_im_conj_mul(acc, l0, p0), p1, l1);
+    return acc;
+}
+
 // c ? a : b, component by component: a select of whole structs can be lowered to a select
 // of their addresses, which would keep register arrays out of registers.
 template <class R>
@@ -608,22 +638,18 @@ __device__ __forceinline__ void reg_sweep_pair(Cx<R> (&vp)[A], Cx<R> (&vl)[A], c
         if constexpr (((j >> P) & 1) == 0) {
             constexpr int j1 = j + (1 << P);
             const Cx<R> p0 = vp[j], p1 = vp[j1], l0 = vl[j], l1 = vl[j1];
-            R t = R(0);
-            if constexpr (G == 1) t = im_conj_mul(l0, p1) + im_conj_mul(l1, p0);
-            if constexpr (G == 2) t = (l1.re * p0.re + l1.im * p0.im) - (l0.re * p1.re + l0.im * p1.im);
-            if constexpr (G == 3) t = im_conj_mul(l0, p0) - im_conj_mul(l1, p1);
             Cx<R> q0, q1, m0, m1;
             apply_form<R, F>(ud, p0, p1, q0, q1);
             apply_form<R, F>(ud, l0, l1, m0, m1);
             if constexpr (SEL) {
                 const bool c = ok && (j & creg) == creg;
-                if constexpr (G != 0) acc += c ? t : R(0);
+                if constexpr (G != 0) acc += c ? add_pair_term<R, G>(R(0), p0, p1, l0, l1) : R(0);
                 vp[j] = csel(c, q0, p0);
                 vp[j1] = csel(c, q1, p1);
                 vl[j] = csel(c, m0, l0);
                 vl[j1] = csel(c, m1, l1);
             } else {
-                if constexpr (G != 0) acc += t;
+                acc = add_pair_term<R, G>(acc, p0, p1, l0, l1);
                 vp[j] = q0;
                 vp[j1] = q1;
                 vl[j] = m0;
@@ -820,7 +846,7 @@ __global__ __launch_bounds__(T) void k_tile_sweep_reg(Cx<R> *__restrict__ phi, C
                             u_sum = R(0);
                             static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
                                 constexpr int j = decltype(jc)::value;
-                                u_sum += im_conj_mul(vl[j], vp[j]);
+                                u_sum = add_im_conj_mul(u_sum, vl[j], vp[j]);
                             });
                             have_u = true;
                         }
