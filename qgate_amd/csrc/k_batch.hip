@@ -436,7 +436,7 @@ BPass make_pass(const Layout &l, int local[64]) {
 }
 
 struct Tuning {
-    int tile_bits = 0;   // forward / Pauli tile bits (the sweep uses one fewer)
+    int tile_bits = 0;   // forward / Pauli tile bits (the sweep: at most BCfg::SweepBits)
     int threads = 256;   // LDS passes
     int grid_per_cu = 4; // LDS passes: persistent workgroups per compute unit
     int registers = 0;   // 1: forward and sweep passes keep their tiles in registers
@@ -583,7 +583,8 @@ void build_group(Group &g, const Tuning &t) {
     // complex64 tiles move two-amplitude units, so they hold bit 0 plus at least one target
     const int min_bits = std::min(n, c64 ? 2 : 1);
     const int mf = std::max(min_bits, std::min(t.tile_bits, n));
-    const int ms = std::max(min_bits, std::min(t.tile_bits - 1, n));
+    // sweep tiles hold two states: at most SweepBits; a group that fits one tile sweeps it whole
+    const int ms = std::max(min_bits, std::min(std::min(t.tile_bits, (int)C::SweepBits), n));
     const int lowf = std::min(C::Low, mf), lows = std::min(C::Low, ms);
     // register-resident passes need full-size tiles (the kernels' compile-time tile bits)
     typedef TileCfg<R> TC;
