@@ -32,7 +32,7 @@ extern "C" {
 enum { QGHIP_OK = 0, QGHIP_ERROR = -1, QGHIP_EVALUE = -2 };
 enum { QGHIP_FP64 = 0, QGHIP_FP32 = 1 };
 enum { QGHIP_MATHOP_NULL = 0, QGHIP_MATHOP_PROB = 1 };
-enum { QGHIP_FUSION_OFF = 0, QGHIP_FUSION_TILE = 1 };
+enum { QGHIP_FUSION_OFF = 0, QGHIP_FUSION_TILE = 1, QGHIP_FUSION_DENSE = 2 };
 
 typedef struct qghip_qubit_states qghip_qubit_states;
 typedef struct qghip_qubit_processor qghip_qubit_processor;
@@ -85,8 +85,12 @@ int qghip_qubit_processor_delete(qghip_qubit_processor *p);
  * vector and applies them in tile passes (many gates per trip through HBM) when anything else
  * touches that state vector, on synchronize, or every 4096 gates. tile_bits (<= 0: default
  * 12 for FP32, 11 for FP64) bounds the tile to 2^tile_bits amplitudes; low_bits (< 0: default
- * 5 / 4) is the number of lowest bits every tile keeps contiguous. QGHIP_FUSION_OFF applies
- * each gate when called, as the reference does. */
+ * 5 / 4) is the number of lowest bits every tile keeps contiguous. QGHIP_FUSION_DENSE queues
+ * the same way and applies runs of consecutive gates on at most tile_bits (2..6; <= 0: 6)
+ * lanes as one dense unitary each, built in FP64 on the host (complex64 blocks of 5 or 6
+ * lanes on FP32 matrix cores, the others on the vector units; a run of one gate on the
+ * streaming gate kernels). QGHIP_FUSION_OFF applies each gate when called, as the reference
+ * does. */
 int qghip_qubit_processor_set_fusion(qghip_qubit_processor *p, int mode, int tile_bits,
                                      int low_bits);
 /* qubit_processor_synchronize (glue.cpp:251-260): completes all queued device work. */

@@ -20,6 +20,7 @@ struct qghip_qubit_states {
     // gates queued by a processor with fusion on, applied in tile passes before anything
     // else reads or writes this state vector
     std::vector<QueuedGate> pending;
+    int fuse_mode = 0;              // mode the pending gates were queued under (TILE or DENSE)
     int tile_bits = 0, low_bits = -1;
 
     size_t n_states() const { return size_t(1) << n_lanes; }
@@ -27,8 +28,8 @@ struct qghip_qubit_states {
 
 struct qghip_qubit_processor {
     Prec prec;
-    int fusion = 0;             // QGHIP_FUSION_OFF or QGHIP_FUSION_TILE
-    int tile_bits = 0;          // <= 0: default for the precision
+    int fusion = 0;             // QGHIP_FUSION_OFF, _TILE or _DENSE
+    int tile_bits = 0;          // <= 0: default for the precision (DENSE: the block width, default 6)
     int low_bits = -1;          // < 0: default
 };
 
@@ -99,6 +100,8 @@ void unlist(qghip_qubit_states *qs) {
     v.erase(std::remove(v.begin(), v.end(), qs), v.end());
 }
 
+constexpr int kDenseWidth = 6;   // default block width of QGHIP_FUSION_DENSE
+
 // Apply the queued gates of qs.
 void flush(qghip_qubit_states *qs) {
     if (qs == nullptr || qs->pending.empty()) return;
@@ -106,6 +109,14 @@ void flush(qghip_qubit_states *qs) {
     gates.swap(qs->pending);
     unlist(qs);
     if (qs->d_ptr == nullptr) return;
+    if (qs->fuse_mode == QGHIP_FUSION_DENSE) {
+        const int width = qs->tile_bits > 0 ? qs->tile_bits : kDenseWidth;
+        if (qs->prec == precFP32)
+            apply_dense_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, width);
+        else
+            apply_dense_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, width);
+        return;
+    }
     if (qs->prec == precFP32)
         apply_gate_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
                                    qs->low_bits);
@@ -167,8 +178,13 @@ void apply_matrix(const qghip_qubit_processor *p, Prec prec, qghip_qubit_states 
     std::sort(sorted.begin(), sorted.end());
     if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
         value_error("duplicate control lanes");
-    if (p != nullptr && p->fusion == QGHIP_FUSION_TILE) {
+    if (p != nullptr && (p->fusion == QGHIP_FUSION_TILE || p->fusion == QGHIP_FUSION_DENSE)) {
+        // gates queued under other settings are applied under those first
+        if (!qs->pending.empty() &&
+            (qs->fuse_mode != p->fusion || qs->tile_bits != p->tile_bits || qs->low_bits != p->low_bits))
+            flush(qs);
         if (qs->pending.empty()) pending_states().push_back(qs);
+        qs->fuse_mode = p->fusion;
         qs->tile_bits = p->tile_bits;
         qs->low_bits = p->low_bits;
         qs->pending.push_back(QueuedGate{m, controls, target});
@@ -303,8 +319,11 @@ int qghip_qubit_processor_delete(qghip_qubit_processor *p) {
 int qghip_qubit_processor_set_fusion(qghip_qubit_processor *p, int mode, int tile_bits, int low_bits) {
     return guarded([&] {
         if (p == nullptr) value_error("null processor handle");
-        if (mode != QGHIP_FUSION_OFF && mode != QGHIP_FUSION_TILE) value_error("unknown fusion mode");
+        if (mode != QGHIP_FUSION_OFF && mode != QGHIP_FUSION_TILE && mode != QGHIP_FUSION_DENSE)
+            value_error("unknown fusion mode");
         if (tile_bits > 14) value_error("tile_bits out of range");
+        if (mode == QGHIP_FUSION_DENSE && (tile_bits == 1 || tile_bits > 6))
+            value_error("dense block width must be 2..6");
         if (low_bits > 8) value_error("low_bits out of range");
         p->fusion = mode;
         p->tile_bits = tile_bits;
@@ -518,6 +537,9 @@ int qghip_qubit_processor_apply_unitary(qghip_qubit_processor *p, qghip_qubit_st
         if (k < 1 || k > 6 || k > qs->n_lanes) value_error("unitary width out of range");
         std::vector<int> ls(lanes, lanes + k);
         for (int l : ls) check_lane(qs, l);
+        std::vector<int> sorted(ls);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) value_error("duplicate unitary lanes");
         flush(qs);
         const int D = 1 << k;
         std::vector<std::complex<double>> m(D * D);

@@ -22,7 +22,10 @@
 // Launch shape: one wavefront per block except for partners 64-128 units away (two); the
 // choice comes from per-target-bit bandwidth sweeps at 30 qubits.
 #include <algorithm>
+#include <complex>
+#include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
@@ -351,16 +354,112 @@ __global__ __launch_bounds__(kBlock) void k_unitary(Cx<R> *__restrict__ a,
     }
 }
 
-}  // namespace
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Dense 2^K x 2^K unitary on complex64 amplitudes on FP32 matrix cores (K = 5, 6), where the
+// product's 8 x 2^K flops per amplitude pass the vector units' rate. y = U x is the real
+// product [yr; yi] = W [xr; xi], W = [[Ur, -Ui], [Ui, Ur]] (M x M, M = 2^(K+1)), with one
+// column per group of 2^K amplitudes. A wavefront takes 16 groups at a time and accumulates
+// the M / 16 output blocks of 16 rows x 16 groups with v_mfma_f32_16x16x4f32, four input rows
+// per instruction:
+//   A operand, lane l: W[16 t + l % 16][4 kk + l / 16] from LDS (W stored column-major with
+//     a padded column stride, so the four 16-lane fragments of a read fall on distinct banks);
+//   B operand, lane l: row 4 kk + l / 16 of group l % 16, from registers (lane l loads the
+//     amplitudes c = l / 16 (mod 4) of its group);
+//   result element v of lane l: row 16 t + 4 (l / 16) + v of group l % 16; blocks t and
+//     t + M / 32 hold the real and imaginary part of the same amplitude.
+// Persistent workgroups stride over the tasks, so W is staged into LDS once per workgroup.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<float> *__restrict__ a,
+                                                         const float *__restrict__ wcol,
+                                                         const UnitaryArgs<float> g, u64 n_tasks) {
+    constexpr int D = 1 << K, M = 2 * D, NT = M / 16, MS = M + 16;
+    __shared__ float w[M * MS];   // w[c * MS + r] = W[r][c]
+    for (int i = threadIdx.x; i < M * M; i += kBlock) w[(i / M) * MS + i % M] = wcol[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, p = lane >> 4;
+    for (u64 task = (u64)blockIdx.x * (kBlock / 64) + wave; task < n_tasks; task += (u64)gridDim.x * (kBlock / 64)) {
+        const u64 base = insert_zero_bits(task * 16 + j, g.ins);
+        // x_c for c = 4 q + p
+        float xr[D / 4], xi[D / 4];
+#pragma unroll
+        for (int q = 0; q < D / 4; ++q) {
+            const u64 m0 = g.member[4 * q], m1 = g.member[4 * q + 1], m2 = g.member[4 * q + 2],
+                      m3 = g.member[4 * q + 3];
+            const u64 off = p == 0 ? m0 : p == 1 ? m1 : p == 2 ? m2 : m3;
+            const Cx<float> v = a[base | off];
+            xr[q] = v.re;
+            xi[q] = v.im;
+        }
+        v4f acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[t][v] = 0.f;
+#pragma unroll
+        for (int kk = 0; kk < M / 4; ++kk) {
+            const float b = kk < D / 4 ? xr[kk] : xi[kk - D / 4];
+            const float *col = w + (4 * kk + p) * MS + j;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(col[16 * t], b, acc[t], 0, 0, 0);
+            // one step's matrix reads in flight at a time (hoisting them all would take every
+            // register of the wave)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int t = 0; t < NT / 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * t + v;   // + 4 p
+                const u64 m0 = g.member[r], m1 = g.member[r + 4], m2 = g.member[r + 8], m3 = g.member[r + 12];
+                const u64 off = p == 0 ? m0 : p == 1 ? m1 : p == 2 ? m2 : m3;
+                a[base | off] = cx<float>(acc[t][v], acc[t + NT / 2][v]);
+            }
+    }
+}
+
+// Workgroups per compute unit of the matrix-core kernel (LDS: 20 KiB for K = 5, 72 KiB for 6).
+constexpr int mfma_groups_per_cu(int k) { return k == 6 ? 2 : 4; }
+
+// The matrix-core kernel takes complex64 blocks of 5 or 6 qubits on states of at least 16
+// groups; everything else takes the vector kernel.
+template <class R>
+bool use_mfma(int k, int n_lanes) {
+    return std::is_same<R, float>::value && (k == 5 || k == 6) && n_lanes - k >= 4;
+}
+
+// Device matrix of a block: complex row-major for the vector kernel (D x D Cx<R>), the real
+// form W column-major for the matrix-core kernel (M x M floats). Bytes of either.
+template <class R>
+size_t unitary_bytes(int k, int n_lanes) {
+    const size_t D = size_t(1) << k;
+    return use_mfma<R>(k, n_lanes) ? sizeof(float) * 4 * D * D : sizeof(Cx<R>) * D * D;
+}
 
 template <class R>
-void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
-                    const std::vector<std::complex<double>> &mat) {
+void pack_unitary(int k, int n_lanes, const std::vector<std::complex<double>> &mat, char *out) {
+    const int D = 1 << k;
+    if (use_mfma<R>(k, n_lanes)) {
+        const int M = 2 * D;
+        float *w = reinterpret_cast<float *>(out);
+        for (int r = 0; r < M; ++r)
+            for (int c = 0; c < M; ++c) {
+                const std::complex<double> u = mat[(size_t)(r % D) * D + (c % D)];
+                const float re = (float)u.real(), im = (float)u.imag();
+                w[(size_t)c * M + r] = (r < D) == (c < D) ? re : (r < D ? -im : im);
+            }
+        return;
+    }
+    Cx<R> *m = reinterpret_cast<Cx<R> *>(out);
+    for (int i = 0; i < D * D; ++i) m[i] = cx<R>((R)mat[i].real(), (R)mat[i].imag());
+}
+
+template <class R>
+UnitaryArgs<R> unitary_args(int n_lanes, const std::vector<int> &lanes) {
     const int k = (int)lanes.size();
     QG_REQUIRE(1 <= k && k <= kMaxFuse, "unitary width %d out of range", k);
     QG_REQUIRE(k <= n_lanes, "unitary wider than the state");
-    const int D = 1 << k;
-    QG_REQUIRE((int)mat.size() == D * D, "unitary matrix has wrong size");
     UnitaryArgs<R> g;
     std::vector<int> sorted(lanes);
     std::sort(sorted.begin(), sorted.end());
@@ -370,19 +469,39 @@ void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
         QG_REQUIRE(i == 0 || sorted[i] != sorted[i - 1], "duplicate lane in unitary");
         g.ins.pos[i] = sorted[i];
     }
-    for (int c = 0; c < D; ++c) {
+    for (int c = 0; c < (1 << k); ++c) {
         u64 off = 0;
         for (int j = 0; j < k; ++j)
             if (c & (1 << j)) off |= u64(1) << lanes[j];
         g.member[c] = off;
     }
+    for (int c = 1 << k; c < (1 << kMaxFuse); ++c) g.member[c] = 0;
     g.k = k;
     g.n_groups = u64(1) << (n_lanes - k);
-    std::vector<Cx<R>> hm(D * D);
-    for (int i = 0; i < D * D; ++i) hm[i] = cx<R>((R)mat[i].real(), (R)mat[i].imag());
-    Cx<R> *dm = static_cast<Cx<R> *>(dev_alloc(sizeof(Cx<R>) * D * D));
+    return g;
+}
+
+// Launch one block whose matrix is already on the device (packed by pack_unitary).
+template <class R>
+void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, const void *dmat) {
+    const UnitaryArgs<R> g = unitary_args<R>(n_lanes, lanes);
+    const int k = g.k;
     hipStream_t s = stream();
-    QG_HIP(hipMemcpyAsync(dm, hm.data(), sizeof(Cx<R>) * D * D, hipMemcpyHostToDevice, s));
+    if constexpr (std::is_same<R, float>::value) {
+        if (use_mfma<R>(k, n_lanes)) {
+            const u64 n_tasks = g.n_groups / 16;
+            const int per_cu = mfma_groups_per_cu(k);
+            const unsigned grid =
+                (unsigned)std::min<u64>((n_tasks + 3) / 4, (u64)std::max(1, runtime().n_cus * per_cu));
+            const float *w = static_cast<const float *>(dmat);
+            if (k == 5)
+                hipLaunchKernelGGL((k_unitary_mfma<5>), grid, kBlock, 0, s, a, w, g, n_tasks);
+            else
+                hipLaunchKernelGGL((k_unitary_mfma<6>), grid, kBlock, 0, s, a, w, g, n_tasks);
+            return;
+        }
+    }
+    const Cx<R> *dm = static_cast<const Cx<R> *>(dmat);
     for_each_piece(g.n_groups, [&](u64 first, u64 count) {
         const unsigned grid = grid_for(count, kBlock);
         switch (k) {
@@ -394,14 +513,135 @@ void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
         case 6: hipLaunchKernelGGL((k_unitary<R, 6>), grid, kBlock, 0, s, a, dm, g, first); break;
         }
     });
-    QG_HIP(hipGetLastError());
-    QG_HIP(hipStreamSynchronize(s));   // hm is a host temporary
-    dev_free(dm, sizeof(Cx<R>) * D * D);
+}
+
+// Unitary of a run of gates on `lanes` (matrix index bit j <-> lanes[j]), in FP64: each
+// gate, in order, on every column of the running product (the pair update of
+// CPUQubitProcessor.cpp:307-362 on a 2^k-amplitude register).
+std::vector<std::complex<double>> block_unitary(const std::vector<int> &lanes,
+                                                const std::vector<const QueuedGate *> &gates) {
+    const int k = (int)lanes.size(), D = 1 << k;
+    std::vector<std::complex<double>> u((size_t)D * D, 0.);
+    for (int i = 0; i < D; ++i) u[(size_t)i * D + i] = 1.;
+    auto bit_of = [&](int lane) {
+        return (int)(std::find(lanes.begin(), lanes.end(), lane) - lanes.begin());
+    };
+    for (const QueuedGate *g : gates) {
+        const int t = bit_of(g->target);
+        int cmask = 0;
+        for (int c : g->ctrl) cmask |= 1 << bit_of(c);
+        const auto &m = g->m.m;
+        for (int i0 = 0; i0 < D; ++i0) {
+            if ((i0 >> t) & 1 || (i0 & cmask) != cmask) continue;
+            const int i1 = i0 | (1 << t);
+            for (int c = 0; c < D; ++c) {
+                const std::complex<double> x0 = u[(size_t)i0 * D + c], x1 = u[(size_t)i1 * D + c];
+                u[(size_t)i0 * D + c] = m[0][0] * x0 + m[0][1] * x1;
+                u[(size_t)i1 * D + c] = m[1][0] * x0 + m[1][1] * x1;
+            }
+        }
+    }
+    return u;
+}
+
+}  // namespace
+
+template <class R>
+void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
+                    const std::vector<std::complex<double>> &mat) {
+    const int k = (int)lanes.size();
+    QG_REQUIRE(1 <= k && k <= kMaxFuse, "unitary width %d out of range", k);
+    const int D = 1 << k;
+    QG_REQUIRE((int)mat.size() == D * D, "unitary matrix has wrong size");
+    unitary_args<R>(n_lanes, lanes);   // checks the lanes before anything is allocated
+    const size_t bytes = unitary_bytes<R>(k, n_lanes);
+    std::vector<char> hm(bytes);
+    pack_unitary<R>(k, n_lanes, mat, hm.data());
+    void *dm = dev_alloc(bytes);
+    hipStream_t s = stream();
+    hipError_t e = hipMemcpyAsync(dm, hm.data(), bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        launch_unitary_dev<R>(a, n_lanes, lanes, dm);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(s);   // hm is a host temporary
+    dev_free(dm, bytes);
+    QG_HIP(e);
+    QG_HIP(e2);
+}
+
+template <class R>
+void apply_dense_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int width) {
+    QG_REQUIRE(2 <= width && width <= kMaxFuse, "dense block width %d out of range", width);
+    // plan: greedy blocks of consecutive gates on at most `width` lanes
+    struct Block {
+        std::vector<int> lanes;
+        std::vector<const QueuedGate *> gates;
+    };
+    std::vector<Block> blocks;
+    Block cur;
+    for (const QueuedGate &g : gates) {
+        std::vector<int> need(g.ctrl);
+        need.push_back(g.target);
+        std::vector<int> merged(cur.lanes);
+        for (int l : need)
+            if (std::find(merged.begin(), merged.end(), l) == merged.end()) merged.push_back(l);
+        if ((int)merged.size() > width && !cur.gates.empty()) {
+            blocks.push_back(cur);
+            cur = Block();
+            merged = need;
+        }
+        cur.lanes = merged;
+        cur.gates.push_back(&g);
+        if ((int)need.size() > width) {   // wider than a block: applied alone
+            blocks.push_back(cur);
+            cur = Block();
+        }
+    }
+    if (!cur.gates.empty()) blocks.push_back(cur);
+    // matrices of the multi-gate blocks, uploaded together
+    std::vector<size_t> offset(blocks.size(), 0);
+    size_t total = 0;
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        if (blocks[b].gates.size() < 2) continue;
+        offset[b] = total;
+        total += (unitary_bytes<R>((int)blocks[b].lanes.size(), n_lanes) + 255) & ~size_t(255);
+    }
+    std::vector<char> host(std::max<size_t>(total, 1));
+    for (size_t b = 0; b < blocks.size(); ++b)
+        if (blocks[b].gates.size() >= 2)
+            pack_unitary<R>((int)blocks[b].lanes.size(), n_lanes, block_unitary(blocks[b].lanes, blocks[b].gates),
+                            host.data() + offset[b]);
+    char *dev = total > 0 ? static_cast<char *>(dev_alloc(total)) : nullptr;
+    hipStream_t s = stream();
+    hipError_t e = total > 0 ? hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, s) : hipSuccess;
+    try {
+        for (size_t b = 0; b < blocks.size() && e == hipSuccess; ++b) {
+            const Block &bl = blocks[b];
+            if (bl.gates.size() == 1) {
+                const QueuedGate &g = *bl.gates[0];
+                launch_gate<R>(a, n_lanes, g.m, g.ctrl, g.target);
+            } else {
+                launch_unitary_dev<R>(a, n_lanes, bl.lanes, dev + offset[b]);
+            }
+            e = hipGetLastError();
+        }
+    } catch (...) {
+        hipStreamSynchronize(s);
+        if (dev) dev_free(dev, total);
+        throw;
+    }
+    const hipError_t e2 = hipStreamSynchronize(s);   // the matrices are host temporaries
+    if (dev) dev_free(dev, total);
+    QG_HIP(e);
+    QG_HIP(e2);
 }
 
 template void launch_unitary<float>(Cx<float> *, int, const std::vector<int> &,
                                     const std::vector<std::complex<double>> &);
 template void launch_unitary<double>(Cx<double> *, int, const std::vector<int> &,
                                      const std::vector<std::complex<double>> &);
+template void apply_dense_sequence<float>(Cx<float> *, int, const std::vector<QueuedGate> &, int);
+template void apply_dense_sequence<double>(Cx<double> *, int, const std::vector<QueuedGate> &, int);
 
 }  // namespace qg

@@ -36,6 +36,12 @@ template <class R>
 void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int max_bits,
                          int low_bits);
 
+// Apply `gates` in order as dense unitaries of runs of consecutive gates on at most `width`
+// (2..6) lanes, each built in FP64 on the host (k_gates.hip); a run of one gate, or a gate on
+// more lanes than `width`, goes to the streaming gate kernels.
+template <class R>
+void apply_dense_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int width);
+
 // One step of the adjoint backward sweep: U^dagger (FP64) with its controls and target, and
 // the generator of a rotation (pauli 1 X, 2 Y, 3 Z; 0 for a fixed gate).
 struct SweepGate {

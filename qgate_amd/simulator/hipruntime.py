@@ -140,7 +140,7 @@ def _handles(objs):
 this.initialized = False
 this.device_id = 0
 this.native_instances = weakref.WeakValueDictionary()
-_FUSION_MODES = {'off': 0, 'tile': 1}
+_FUSION_MODES = {'off': 0, 'tile': 1, 'dense': 2}
 this.fusion = ('off', 0, -1)
 
 
@@ -155,7 +155,9 @@ def set_fusion(mode='tile', tile_bits=0, low_bits=-1):
     """Gate fusion of the processors created from now on and of the live ones: 'tile' queues
     the gates of each state vector and applies them in tile passes (many gates per trip
     through HBM; tiles of 2^tile_bits amplitudes whose lowest low_bits bits are contiguous,
-    0 / -1: defaults); 'off' applies every gate when it is issued."""
+    0 / -1: defaults); 'dense' queues them and applies runs of consecutive gates on at most
+    tile_bits (2..6, 0: 6) qubits as one dense unitary each (built in FP64; complex64 blocks
+    of 5 or 6 qubits on matrix cores); 'off' applies every gate when it is issued."""
     if mode not in _FUSION_MODES:
         raise ValueError('fusion mode must be one of {}'.format(sorted(_FUSION_MODES)))
     this.fusion = (mode, int(tile_bits), int(low_bits))

@@ -156,7 +156,7 @@ def test_pieces_with_fusion(hip_module, port_module, dtype):
 
 def test_fusion_mode_checked(hip_module):
     with pytest.raises(ValueError):
-        hip_module.set_fusion('dense')
+        hip_module.set_fusion('blocks')
     with pytest.raises(ValueError):
         hip_module.create_qubit_states(np.float64).processor.set_fusion('tile', 20, -1)
 
