@@ -43,9 +43,15 @@ def _prepared(module, n, dtype, seed):
 
 
 def _read(module, qs, n, dtype):
+    """The amplitudes, index bit i = lane i."""
+    from qgate_amd.simulator.lanes import Lane
+    lanes = []
+    for i in range(n):
+        ln = Lane(None, i)
+        ln.set_external(i)
+        lanes.append(ln)
     out = np.empty(1 << n, np.complex128 if dtype == np.float64 else np.complex64)
-    module.create_qubits_states_getter(dtype).get_states(out, 0, qubits.null, [(qs, list(range(n)))], [],
-                                                         1 << n, 0, 1)
+    module.create_qubits_states_getter(dtype).get_states(out, 0, qubits.null, [(qs, lanes)], [], 1 << n, 0, 1)
     return out.astype(np.complex128)
 
 
@@ -138,9 +144,11 @@ def test_dense_fusion_brickwork_qft_dynamic(hip_module, port_module, dtype):
 
 
 def test_dense_mode_checked(hip_module):
-    proc = hip_module.create_qubit_states(np.float32).processor
+    qs = hip_module.create_qubit_states(np.float32)
+    proc = qs.processor
     for bad in ((1, -1), (7, -1)):
         with pytest.raises(ValueError):
             proc.set_fusion('dense', *bad)
     proc.set_fusion('dense', 0, -1)
     proc.set_fusion('off')
+    qs.delete()
