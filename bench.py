@@ -38,6 +38,11 @@ adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qub
 RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
 once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
 
+fused_dense: the same step with runs of gates fused into dense unitaries of at most k = 2..6
+qubits (built in FP64, applied as one kernel each; complex64 blocks of 5-6 qubits on FP32
+matrix cores), and single 5- and 6-qubit blocks on the 30-qubit state with their GB/s,
+TFLOP/s and fraction of the 157.3 TFLOP/s FP32 matrix-core peak.
+
 cpu_baseline: the reference's own C++ CPU runtime (oracle/_ref, or the C restatement when
 that is not built) applying the first gates of the same circuit to a 30-qubit complex64
 state in host memory, for about 15 s of work, all host cores.
@@ -164,6 +169,45 @@ def roofline(runner, n, amp, reps=4):
             'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
             'by_kernel_GBs': split,
             'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
+
+
+MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
+
+
+def dense_timing(r, gates, step_s, step_bytes, n):
+    """The step with dense-block fusion at widths 2..6 (one warm-up, one timed run each), and
+    single dense blocks of 5 and 6 qubits on the 30-qubit complex64 state (the matrix-core
+    kernel): 2 S 2^n bytes and 8 2^k 2^n flops per block, best of 3 synchronised calls."""
+    from qgate_amd.simulator import hipruntime
+    steps = {}
+    for k in (2, 3, 4, 5, 6):
+        hipruntime.set_fusion('dense', k)
+        r.proc.set_fusion('dense', k)
+        r.apply(gates)
+        r.sync()
+        t0 = time.perf_counter()
+        r.apply(gates)
+        r.sync()
+        dt = time.perf_counter() - t0
+        steps[k] = {'ms_per_step': round(dt * 1e3, 1), 'gate_bytes_per_s_GBs': round(step_bytes / dt / 1e9, 1),
+                    'speedup_vs_unfused': round(step_s / dt, 2)}
+    hipruntime.set_fusion('off')
+    r.proc.set_fusion('off')
+    rs = np.random.RandomState(17)
+    blocks = {}
+    for k, lanes in ((5, [2, 9, 15, 21, 27]), (6, [1, 6, 12, 17, 23, 28])):
+        z = rs.normal(size=(1 << k, 1 << k)) + 1j * rs.normal(size=(1 << k, 1 << k))
+        u, _ = np.linalg.qr(z)
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r.proc.apply_unitary(r.qs, lanes, u)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        tflops = 8 * (1 << k) * (1 << n) / best / 1e12
+        blocks[k] = {'lanes': lanes, 'ms': round(best * 1e3, 3), 'GBs': round(2 * 8 * (1 << n) / best / 1e9, 1),
+                     'TFLOPs': round(tflops, 1), 'frac_of_mfma_fp32_peak': round(tflops / MFMA_FP32_PEAK_TFLOPS, 3)}
+    return {'steps': steps, 'blocks': blocks}
 
 
 def copy_rate_GBs(gib=4, reps=5):
@@ -491,6 +535,8 @@ def main():
     hipruntime.set_fusion('off')
     r.proc.set_fusion('off')
     print('[bench] fused step {:.3f} s'.format(fused_s), file=sys.stderr, flush=True)
+    dense = dense_timing(r, gates, step_s, step_bytes, n)
+    print('[bench] dense fusion: {}'.format(dense), file=sys.stderr, flush=True)
 
     # a Pauli observable on the circuit's output state (before the measurements collapse it)
     pauli = pauli_timing(r, n)
@@ -566,6 +612,7 @@ def main():
         'fused_tile': {'ms_per_step': round(fused_s * 1e3, 2), 'gates_per_s': round(len(gates) / fused_s, 1),
                        'gate_bytes_per_s_GBs': round(step_bytes / fused_s / 1e9, 1),
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
+        'fused_dense': dense,
         'pauli_observable_30q': pauli,
         'readout_30q': readout,
         'adjoint_24q': adjoint,

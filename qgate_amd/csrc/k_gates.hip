@@ -379,19 +379,30 @@ __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<float> *__restrict__
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, p = lane >> 4;
-    for (u64 task = (u64)blockIdx.x * (kBlock / 64) + wave; task < n_tasks; task += (u64)gridDim.x * (kBlock / 64)) {
-        const u64 base = insert_zero_bits(task * 16 + j, g.ins);
-        // x_c for c = 4 q + p
-        float xr[D / 4], xi[D / 4];
+    const u64 stride = (u64)gridDim.x * (kBlock / 64);
+    // member offsets are ORs of per-bit offsets: member[4 q + p] = member[4 q] | member[p] and
+    // member[16 t + 4 p + v] = member[16 t + v] | member[4 p], so each lane holds its own part
+    // (two selects, no branches) and ORs in a uniform part per access
+    const u64 lane_in = ((p & 1) ? g.member[1] : 0) | ((p & 2) ? g.member[2] : 0);
+    const u64 lane_out = ((p & 1) ? g.member[4] : 0) | ((p & 2) ? g.member[8] : 0);
+    // x_c (c = 4 q + p) of group j of a task
+    auto load = [&](u64 task, float (&xr)[D / 4], float (&xi)[D / 4]) __attribute__((always_inline)) {
+        const u64 base = insert_zero_bits(task * 16 + j, g.ins) | lane_in;
 #pragma unroll
         for (int q = 0; q < D / 4; ++q) {
-            const u64 m0 = g.member[4 * q], m1 = g.member[4 * q + 1], m2 = g.member[4 * q + 2],
-                      m3 = g.member[4 * q + 3];
-            const u64 off = p == 0 ? m0 : p == 1 ? m1 : p == 2 ? m2 : m3;
-            const Cx<float> v = a[base | off];
+            const Cx<float> v = a[base | g.member[4 * q]];
             xr[q] = v.re;
             xi[q] = v.im;
         }
+    };
+    u64 task = (u64)blockIdx.x * (kBlock / 64) + wave;
+    float xr[D / 4], xi[D / 4];
+    if (task < n_tasks) load(task, xr, xi);
+    for (; task < n_tasks; task += stride) {
+        // the next task's inputs are in flight while this task's products run
+        float nr[D / 4], ni[D / 4];
+        if (task + stride < n_tasks) load(task + stride, nr, ni);
+        const u64 base = insert_zero_bits(task * 16 + j, g.ins) | lane_out;
         v4f acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
@@ -411,16 +422,18 @@ __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<float> *__restrict__
         for (int t = 0; t < NT / 2; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = 16 * t + v;   // + 4 p
-                const u64 m0 = g.member[r], m1 = g.member[r + 4], m2 = g.member[r + 8], m3 = g.member[r + 12];
-                const u64 off = p == 0 ? m0 : p == 1 ? m1 : p == 2 ? m2 : m3;
-                a[base | off] = cx<float>(acc[t][v], acc[t + NT / 2][v]);
+                a[base | g.member[16 * t + v]] = cx<float>(acc[t][v], acc[t + NT / 2][v]);
             }
+#pragma unroll
+        for (int q = 0; q < D / 4; ++q) {
+            xr[q] = nr[q];
+            xi[q] = ni[q];
+        }
     }
 }
 
 // Workgroups per compute unit of the matrix-core kernel (LDS: 20 KiB for K = 5, 72 KiB for 6).
-constexpr int mfma_groups_per_cu(int k) { return k == 6 ? 2 : 4; }
+constexpr int mfma_groups_per_cu(int k) { return k == 6 ? 2 : 8; }
 
 // The matrix-core kernel takes complex64 blocks of 5 or 6 qubits on states of at least 16
 // groups; everything else takes the vector kernel.
