@@ -49,6 +49,7 @@ state in host memory, for about 15 s of work, all host cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-large]
     python bench.py --batch-only      # the batched adjoint alone, one JSON object
+    python bench.py --dense-only      # dense 5- and 6-qubit blocks alone, one JSON object
 """
 import argparse
 import json
@@ -174,13 +175,13 @@ def roofline(runner, n, amp, reps=4):
 MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
 
 
-def dense_timing(r, gates, step_s, step_bytes, n):
-    """The step with dense-block fusion at widths 2..6 (one warm-up, one timed run each), and
-    single dense blocks of 5 and 6 qubits on the 30-qubit complex64 state (the matrix-core
+def dense_timing(r, gates, step_s, step_bytes, n, widths=(2, 3, 4, 5, 6)):
+    """The step with dense-block fusion at the given widths (one warm-up, one timed run each),
+    and single dense blocks of 5 and 6 qubits on the 30-qubit complex64 state (the matrix-core
     kernel): 2 S 2^n bytes and 8 2^k 2^n flops per block, best of 3 synchronised calls."""
     from qgate_amd.simulator import hipruntime
     steps = {}
-    for k in (2, 3, 4, 5, 6):
+    for k in widths:
         hipruntime.set_fusion('dense', k)
         r.proc.set_fusion('dense', k)
         r.apply(gates)
@@ -481,6 +482,9 @@ def main():
     ap.add_argument('--batch-only', action='store_true',
                     help='only the batched adjoint (tuning run, three graph replays, a short single-circuit '
                          'loop); prints its JSON object (for profiler runs)')
+    ap.add_argument('--dense-only', action='store_true',
+                    help='only the dense 5- and 6-qubit blocks on the 30-qubit state; prints their JSON object '
+                         '(for profiler runs)')
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the per-target gate timing loop and a 2 GiB copy kernel (for counter runs)')
     args = ap.parse_args()
@@ -497,6 +501,11 @@ def main():
     from qgate_amd.simulator import hipruntime
 
     n, amp = args.qubits, 8
+    if args.dense_only:
+        r = Runner(hipruntime, n, np.float32)
+        print(json.dumps(dense_timing(r, [], 1., 0., n, widths=())), flush=True)
+        r.delete()
+        return
     gates = workloads.random_gate_list(n, args.gates, seed=2)
     step_bytes = sum(gate_bytes(k, n, amp) for k, _, _ in gates)
     info = hipruntime.device_info()
