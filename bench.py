@@ -38,6 +38,8 @@ adjoint_24q: forward (<H>) and backward (all 576 gradient entries) of the 24-qub
 RY/RZ/CZ-ring ansatz, each call synchronised; backward_over_forward is the cost of the gradient
 once the forward pass has run, gradient_over_forward that of a gradient call from scratch.
 
+complex128_step: the same unfused step on a complex128 state (16 GiB), one timed run.
+
 fused_dense: the same step with runs of gates fused into dense unitaries of at most k = 2..6
 qubits (built in FP64, applied as one kernel each; complex64 blocks of 5-6 qubits on FP32
 matrix cores), and single 5- and 6-qubit blocks on the 30-qubit state with their GB/s,
@@ -163,16 +165,40 @@ def roofline(runner, n, amp, reps=4):
                       ('t>=7 strided', range(7, n))):
         ms = float(np.mean([per_target[t] for t in rng]))
         split[name] = round(launch_bytes / (ms * 1e-3) / 1e9, 1)
+    # amplitude-bit classes: partner inside the wavefront's 64 amplitudes, inside a 2^13 block,
+    # or further away
+    classes = {}
+    for name, rng in (('t<6', range(0, 6)), ('6<=t<13', range(6, 13)), ('t>=13', range(13, n))):
+        ms = float(np.mean([per_target[t] for t in rng]))
+        classes[name] = round(launch_bytes / (ms * 1e-3) / 1e9, 1)
     return {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
             'traffic': measured_traffic() if (n, amp) == (30, 8) else None,
             'kernel': 'uncontrolled 2x2 gate (k_unit_pair / k_wave_pair / k_strided_pair), complex64, 30 qubits',
             'launch_bytes': launch_bytes, 'mean_launch_ms': round(avg_ms, 4),
             'by_kernel_GBs': split,
+            'by_target_class_GBs': classes,
             'per_target_GBs': [round(launch_bytes / (ms * 1e-3) / 1e9) for ms in per_target]}
 
 
 MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
+
+
+def c128_step_timing(module, n, gates):
+    """The same unfused step on a complex128 state (16 GiB at 30 qubits): one warm-up, one
+    timed run; algorithmic bytes with S = 16."""
+    r = Runner(module, n, np.float64)
+    r.apply(gates)
+    r.sync()
+    t0 = time.perf_counter()
+    r.apply(gates)
+    r.sync()
+    dt = time.perf_counter() - t0
+    r.delete()
+    step_bytes = sum(gate_bytes(k, n, 16) for k, _, _ in gates)
+    return {'ms_per_step': round(dt * 1e3, 1), 'GBs': round(step_bytes / dt / 1e9, 1),
+            'frac_of_hbm_peak': round(step_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
+            'gates_per_s': round(len(gates) / dt, 1)}
 
 
 def dense_timing(r, gates, step_s, step_bytes, n, widths=(2, 3, 4, 5, 6)):
@@ -589,6 +615,8 @@ def main():
     roof = roofline(r, n, amp)
     r.delete()
     roof['copy_GBs'] = copy_rate_GBs()
+    c128 = c128_step_timing(hipruntime, n, gates)
+    print('[bench] complex128 step: {}'.format(c128), file=sys.stderr, flush=True)
     roof['frac_of_copy'] = round(roof['achieved'] / roof['copy_GBs'], 4)
     print('[bench] roofline: {:.1f} GB/s, device copy {:.1f} GB/s'.format(roof['achieved'], roof['copy_GBs']),
           file=sys.stderr, flush=True)
@@ -622,6 +650,7 @@ def main():
                        'gate_bytes_per_s_GBs': round(step_bytes / fused_s / 1e9, 1),
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
         'fused_dense': dense,
+        'complex128_step': c128,
         'pauli_observable_30q': pauli,
         'readout_30q': readout,
         'adjoint_24q': adjoint,
