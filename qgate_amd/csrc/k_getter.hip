@@ -210,25 +210,36 @@ struct ProbSrc {};
 template <class R, bool FROM_STATE>
 __global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
                                                       double *__restrict__ p, double *__restrict__ e_inc,
-                                                      double *__restrict__ t_tot, i64 *last_nonzero) {
+                                                      double *__restrict__ t_tot, i64 *__restrict__ t_last) {
     __shared__ double sp[kTile + kTile / kChunk];   // one pad slot per chunk
     __shared__ double sc[kTileChunks];
+    __shared__ i64 sl[kBlock / 64];
     const i64 base = (i64)blockIdx.x * kTile;
+    // every load of the thread first (kChunk of them in flight), then the FP64 probabilities
+    double v[kChunk];
+    if constexpr (FROM_STATE) {
+        Cx<R> x[kChunk];
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) {
+            const i64 bin = base + r * kBlock + threadIdx.x;
+            x[r] = bin < n_bins ? a[bin] : cx<R>(0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) v[r] = abs2d(x[r]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < kChunk; ++r) {
+            const i64 bin = base + r * kBlock + threadIdx.x;
+            v[r] = bin < n_bins ? p[bin] : 0.;
+        }
+    }
+#pragma unroll
     for (int r = 0; r < kChunk; ++r) {
         const int li = r * kBlock + threadIdx.x;
         const i64 bin = base + li;
-        double v = 0.;
-        if (bin < n_bins) {
-            if (FROM_STATE) {
-                v = abs2d(a[bin]);
-                p[bin] = v;
-            } else {
-                v = p[bin];
-            }
-        } else {
-            p[bin] = 0.;   // padding up to whole tiles reads as zero probability
-        }
-        sp[li + li / kChunk] = v;
+        // probabilities from the state, and zeros for the padding up to whole tiles
+        if (FROM_STATE || bin >= n_bins) p[bin] = v[r];
+        sp[li + li / kChunk] = v[r];
     }
     __syncthreads();
     // serial sum over this thread's chunk
@@ -237,21 +248,42 @@ __global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ 
     i64 last = -1;
     for (int k = 0; k < kChunk; ++k) {
         const int li = c * kChunk + k;
-        const double v = sp[li + li / kChunk];
-        s = (k == 0) ? v : s + v;
-        if (v > 0.) last = base + li;
+        const double x = sp[li + li / kChunk];
+        s = (k == 0) ? x : s + x;
+        if (x > 0.) last = base + li;
     }
     sc[c] = s;
-    // stored as last + 1 so that 0 means "no nonzero bin"
-    if (last >= 0) atomicMax(reinterpret_cast<unsigned long long *>(last_nonzero), (unsigned long long)(last + 1));
+    // the tile's last nonzero bin: maximum over the wavefront, then over the waves (below)
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
+    if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = last;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double acc = 0.;
-        for (int i = 0; i < kTileChunks; ++i) {
-            acc = (i == 0) ? sc[0] : acc + sc[i];
-            sc[i] = acc;
+    // inclusive scan of the chunk totals by one wavefront, in a fixed order: each lane scans
+    // four consecutive chunks, the lanes' totals are scanned across the wavefront (inclusive,
+    // doubling distances), and each lane adds the total of the lanes before it
+    static_assert(kTileChunks == 4 * 64, "one wavefront scans the chunk totals, four per lane");
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        const double q0 = sc[4 * l], q1 = q0 + sc[4 * l + 1], q2 = q1 + sc[4 * l + 2], q3 = q2 + sc[4 * l + 3];
+        double incl = q3;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(incl, (unsigned)d, 64);
+            if (l >= d) incl += y;
         }
-        t_tot[blockIdx.x] = acc;
+        double before = __shfl_up(incl, 1u, 64);
+        if (l == 0) before = 0.;
+        sc[4 * l] = before + q0;
+        sc[4 * l + 1] = before + q1;
+        sc[4 * l + 2] = before + q2;
+        sc[4 * l + 3] = before + q3;
+        if (l == 63) t_tot[blockIdx.x] = incl;
+        if (l == 0) {
+            i64 m = sl[0];
+#pragma unroll
+            for (int w = 1; w < kBlock / 64; ++w) m = max(m, sl[w]);
+            t_last[blockIdx.x] = m;
+        }
     }
     __syncthreads();
     e_inc[(i64)blockIdx.x * kTileChunks + c] = sc[c];
@@ -259,7 +291,8 @@ __global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ 
 
 // One wave per super-tile: serial inclusive scan of the tile totals in place; super total out.
 __global__ __launch_bounds__(64) void k_pool_super(double *__restrict__ m_inc, i64 n_tiles,
-                                                   double *__restrict__ s_tot) {
+                                                   double *__restrict__ s_tot, const i64 *__restrict__ t_last,
+                                                   i64 *__restrict__ s_last) {
     __shared__ double st[kSuperTiles];
     const i64 base = (i64)blockIdx.x * kSuperTiles;
     for (int i = threadIdx.x; i < kSuperTiles; i += 64)
@@ -276,16 +309,26 @@ __global__ __launch_bounds__(64) void k_pool_super(double *__restrict__ m_inc, i
     __syncthreads();
     for (int i = threadIdx.x; i < kSuperTiles; i += 64)
         if (base + i < n_tiles) m_inc[base + i] = st[i];
+    // the super-tile's last nonzero bin
+    i64 last = -1;
+    for (int i = threadIdx.x; i < kSuperTiles; i += 64)
+        if (base + i < n_tiles) last = max(last, t_last[base + i]);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
+    if (threadIdx.x == 0) s_last[blockIdx.x] = last;
 }
 
 // Serial inclusive scan of the super-tile totals in place; the last value is the total.
-__global__ void k_pool_top(double *h_inc, i64 n_supers, double *total) {
+__global__ void k_pool_top(double *h_inc, i64 n_supers, double *total, const i64 *s_last, i64 *last_nonzero) {
     double acc = 0.;
+    i64 last = -1;
     for (i64 i = 0; i < n_supers; ++i) {
         acc = (i == 0) ? h_inc[0] : acc + h_inc[i];
         h_inc[i] = acc;
+        last = max(last, s_last[i]);
     }
     *total = acc;
+    *last_nonzero = last + 1;   // 0: no nonzero bin
 }
 
 struct SampleArgs {
@@ -353,7 +396,7 @@ void pool_alloc(PoolData &pd, int n_lanes) {
     pd.n_supers = (pd.n_tiles + kSuperTiles - 1) / kSuperTiles;
     const size_t n_p = (size_t)pd.n_tiles * kTile;
     const size_t n_m = (size_t)pd.n_supers * kSuperTiles;
-    pd.bytes = sizeof(double) * (n_p + pd.n_chunks + n_m + pd.n_supers + 8);
+    pd.bytes = sizeof(double) * (n_p + pd.n_chunks + n_m + pd.n_supers + 8) + sizeof(i64) * (pd.n_tiles + pd.n_supers);
     pd.block = dev_alloc(pd.bytes);
     double *d = static_cast<double *>(pd.block);
     pd.p = d;
@@ -361,6 +404,8 @@ void pool_alloc(PoolData &pd, int n_lanes) {
     pd.m_inc = pd.e_inc + pd.n_chunks;
     pd.h_inc = pd.m_inc + n_m;
     pd.d_last_nonzero = reinterpret_cast<i64 *>(pd.h_inc + pd.n_supers);
+    pd.t_last = reinterpret_cast<i64 *>(pd.h_inc + pd.n_supers + 8);
+    pd.s_last = pd.t_last + pd.n_tiles;
 }
 
 void pool_free(PoolData &pd) {
@@ -374,13 +419,13 @@ template <class R, bool FROM_STATE>
 void pool_build(PoolData &pd, const Cx<R> *a) {
     hipStream_t s = stream();
     double *d_total = pd.h_inc + pd.n_supers + 1;
-    QG_HIP(hipMemsetAsync(pd.d_last_nonzero, 0, sizeof(i64), s));
     hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE>), (unsigned)pd.n_tiles, kBlock, 0, s, a, pd.n,
-                       pd.p, pd.e_inc, pd.m_inc, pd.d_last_nonzero);
+                       pd.p, pd.e_inc, pd.m_inc, pd.t_last);
     QG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc);
+    hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc, pd.t_last,
+                       pd.s_last);
     QG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pool_top, 1, 1, 0, s, pd.h_inc, pd.n_supers, d_total);
+    hipLaunchKernelGGL(k_pool_top, 1, 1, 0, s, pd.h_inc, pd.n_supers, d_total, pd.s_last, pd.d_last_nonzero);
     QG_HIP(hipGetLastError());
     Runtime &rt = runtime();
     QG_HIP(hipMemcpyAsync(rt.h_pinned, d_total, sizeof(double), hipMemcpyDeviceToHost, s));

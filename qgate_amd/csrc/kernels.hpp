@@ -121,7 +121,9 @@ struct PoolData {
     double *e_inc = nullptr;   // per 16-bin chunk: inclusive prefix within its tile
     double *m_inc = nullptr;   // per 4096-bin tile: inclusive prefix within its super-tile
     double *h_inc = nullptr;   // per 2^22-bin super-tile: inclusive prefix over the pool
-    i64 *d_last_nonzero = nullptr;
+    i64 *d_last_nonzero = nullptr;   // last nonzero bin + 1 (0: none)
+    i64 *t_last = nullptr;           // per tile / super-tile: last nonzero bin, -1 if none
+    i64 *s_last = nullptr;
     i64 n_chunks = 0, n_tiles = 0, n_supers = 0;
     double total = 0., norm = 0.;
     i64 last_nonzero = -1;
