@@ -299,7 +299,7 @@ def device_copy_ms(module, n):
     return {'bytes_each_way': 8 << n, 'ms': round(ms, 3)}
 
 
-def adjoint_timing(dtype, reps=3):
+def adjoint_timing(dtype, reps=9):
     """Forward (<H>) and backward (full gradient) times of the 24-qubit, 12-layer ansatz
     (576 parameters; H = sum a_i Z_i Z_i+1 + sum b_i X_i, a, b ~ N(0, 1), seed 4)."""
     from qgate_amd import workloads

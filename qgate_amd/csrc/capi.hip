@@ -22,6 +22,10 @@ struct qghip_qubit_states {
     std::vector<QueuedGate> pending;
     int fuse_mode = 0;              // mode the pending gates were queued under (TILE or DENSE)
     int tile_bits = 0, low_bits = -1;
+    // a collapse recorded by decohere and not yet launched (dec_lane < 0: none). It precedes
+    // the pending gates; a calc_probability that comes next runs it in the same pass.
+    int dec_lane = -1, dec_value = 0;
+    double dec_norm = 0.;
 
     size_t n_states() const { return size_t(1) << n_lanes; }
 };
@@ -89,7 +93,7 @@ void check_pauli_lanes(const qghip_qubit_states *qs, uint64_t x, uint64_t z) {
     if (qs->n_lanes < 64 && ((x | z) >> qs->n_lanes) != 0) value_error("Pauli lane out of range");
 }
 
-// State vectors with queued gates (so that synchronize can apply them all).
+// State vectors with queued work (so that synchronize can apply it all).
 std::vector<qghip_qubit_states *> &pending_states() {
     static std::vector<qghip_qubit_states *> v;
     return v;
@@ -102,13 +106,26 @@ void unlist(qghip_qubit_states *qs) {
 
 constexpr int kDenseWidth = 6;   // default block width of QGHIP_FUSION_DENSE
 
-// Apply the queued gates of qs.
+bool queued(const qghip_qubit_states *qs) { return !qs->pending.empty() || qs->dec_lane >= 0; }
+
+// Apply the queued work of qs: the recorded collapse, then the queued gates.
 void flush(qghip_qubit_states *qs) {
-    if (qs == nullptr || qs->pending.empty()) return;
+    if (qs == nullptr || !queued(qs)) return;
     std::vector<QueuedGate> gates;
     gates.swap(qs->pending);
+    const int dec_lane = qs->dec_lane;
+    qs->dec_lane = -1;
     unlist(qs);
     if (qs->d_ptr == nullptr) return;
+    if (dec_lane >= 0) {
+        if (qs->prec == precFP32)
+            launch_decohere<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, dec_lane, qs->dec_value,
+                                   (float)qs->dec_norm);
+        else
+            launch_decohere<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, dec_lane, qs->dec_value,
+                                    qs->dec_norm);
+    }
+    if (gates.empty()) return;
     if (qs->fuse_mode == QGHIP_FUSION_DENSE) {
         const int width = qs->tile_bits > 0 ? qs->tile_bits : kDenseWidth;
         if (qs->prec == precFP32)
@@ -125,10 +142,11 @@ void flush(qghip_qubit_states *qs) {
                                     qs->low_bits);
 }
 
-// Forget the queued gates of qs (its contents are about to be overwritten or freed).
+// Forget the queued work of qs (its contents are about to be overwritten or freed).
 void drop(qghip_qubit_states *qs) {
-    if (qs == nullptr || qs->pending.empty()) return;
+    if (qs == nullptr || !queued(qs)) return;
     qs->pending.clear();
+    qs->dec_lane = -1;
     unlist(qs);
 }
 
@@ -183,7 +201,7 @@ void apply_matrix(const qghip_qubit_processor *p, Prec prec, qghip_qubit_states 
         if (!qs->pending.empty() &&
             (qs->fuse_mode != p->fusion || qs->tile_bits != p->tile_bits || qs->low_bits != p->low_bits))
             flush(qs);
-        if (qs->pending.empty()) pending_states().push_back(qs);
+        if (!queued(qs)) pending_states().push_back(qs);
         qs->fuse_mode = p->fusion;
         qs->tile_bits = p->tile_bits;
         qs->low_bits = p->low_bits;
@@ -377,6 +395,18 @@ int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit
         if (p == nullptr || prob == nullptr) value_error("null argument");
         check_qs(qs, p->prec);
         check_lane(qs, local_lane);
+        if (qs->dec_lane >= 0 && qs->pending.empty()) {
+            // the recorded collapse and this probability in one pass over the state
+            const int dec_lane = qs->dec_lane;
+            qs->dec_lane = -1;
+            unlist(qs);
+            by_prec(p->prec, [&](auto r) {
+                typedef decltype(r) R;
+                *prob = decohere_prob<R>(amps<R>(qs), qs->n_lanes, dec_lane, qs->dec_value, (R)qs->dec_norm,
+                                         local_lane);
+            });
+            return;
+        }
         flush(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
@@ -420,12 +450,13 @@ int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double p
         check_lane(qs, local_lane);
         if (value != 0 && value != 1) value_error("value must be 0 or 1");
         flush(qs);
-        // norm in FP64, rounded to the state precision (CPUQubitProcessor.cpp:227, :235)
-        const double norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
-        by_prec(p->prec, [&](auto r) {
-            typedef decltype(r) R;
-            launch_decohere<R>(amps<R>(qs), qs->n_lanes, local_lane, value, (R)norm);
-        });
+        // norm in FP64, rounded to the state precision (CPUQubitProcessor.cpp:227, :235). The
+        // collapse is recorded and launched by the next call that touches qs (flush), or
+        // together with the probability when that call is calc_probability.
+        qs->dec_lane = local_lane;
+        qs->dec_value = value;
+        qs->dec_norm = value == 0 ? 1. / std::sqrt(prob) : 1. / std::sqrt(1. - prob);
+        pending_states().push_back(qs);
     });
 }
 

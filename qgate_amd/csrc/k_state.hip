@@ -101,6 +101,22 @@ __global__ __launch_bounds__(kWave) void k_decohere(Cx<R> *__restrict__ a, BitIn
         store_unit(a, w, v);
         return;
     }
+    if (ins.n == 0) {
+        // unit bit `dist` below the wavefront's 64 units: thread w owns unit w and scales or
+        // clears it, so each store instruction writes whole lines (a keep/drop pair per
+        // thread wrote half lines twice, at about twice the time)
+        Amps<R> v;
+        if (((w & dist) != 0) == (value != 0)) {
+            v = load_unit(a, w);
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
+        } else {
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(0, 0);
+        }
+        store_unit(a, w, v);
+        return;
+    }
     const u64 u0 = insert_zero_bits(w, ins);
     const u64 keep = value ? (u0 | dist) : u0, drop = value ? u0 : (u0 | dist);
     Amps<R> v = load_unit(a, keep), z;
@@ -111,6 +127,64 @@ __global__ __launch_bounds__(kWave) void k_decohere(Cx<R> *__restrict__ a, BitIn
     }
     store_unit(a, keep, v);
     store_unit(a, drop, z);
+}
+
+// k_decohere's collapse (same three forms, each thread taking kProbUnitsPerThread items) and,
+// on the collapsed amplitudes as stored, partial sums of |a|^2 over the amplitudes whose unit
+// index has no bit of `p_unit` set and whose in-unit index has no bit of `p_amp` set: the
+// probability of the next lane, in the same pass.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_decohere_prob(Cx<R> *__restrict__ a, BitInsert ins, u64 dist,
+                                                          int value, R norm, u64 n_items, u64 p_unit,
+                                                          int p_amp, double *partial) {
+    double s = 0.;
+    const u64 base = (u64)blockIdx.x * (kBlock * kProbUnitsPerThread) + threadIdx.x;
+    auto add = [&](u64 unit, const Amps<R> &v) {
+        if ((unit & p_unit) != 0) return;
+#pragma unroll
+        for (int j = 0; j < Unit<R>::W; ++j)
+            if ((j & p_amp) == 0) s += abs2d(v.v[j]);
+    };
+#pragma unroll 4
+    for (int r = 0; r < kProbUnitsPerThread; ++r) {
+        const u64 w = base + (u64)r * kBlock;
+        if (w >= n_items) break;
+        Amps<R> v;
+        if (dist == 0) {
+            v = load_unit(a, w);
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j)
+                v.v[j] = (j == value) ? cx<R>(v.v[j].re * norm, v.v[j].im * norm) : cx<R>(0, 0);
+            store_unit(a, w, v);
+            add(w, v);
+        } else if (ins.n == 0) {
+            if (((w & dist) != 0) == (value != 0)) {
+                v = load_unit(a, w);
+#pragma unroll
+                for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
+                add(w, v);
+            } else {
+#pragma unroll
+                for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(0, 0);
+            }
+            store_unit(a, w, v);
+        } else {
+            const u64 u0 = insert_zero_bits(w, ins);
+            const u64 keep = value ? (u0 | dist) : u0, drop = value ? u0 : (u0 | dist);
+            Amps<R> z;
+            v = load_unit(a, keep);
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j) {
+                v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
+                z.v[j] = cx<R>(0, 0);
+            }
+            store_unit(a, keep, v);
+            store_unit(a, drop, z);
+            add(keep, v);
+        }
+    }
+    s = block_sum<kBlock>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // dst[k] = norm * src[k with a `value` bit inserted at `lane`].
@@ -349,11 +423,38 @@ void launch_set_zero_state(Cx<R> *a, int n_lanes) {
     QG_HIP(hipGetLastError());
 }
 
+namespace {
+
+u64 prob_blocks(u64 n_items) {
+    const u64 per_block = (u64)kBlock * kProbUnitsPerThread;
+    return (n_items + per_block - 1) / per_block;
+}
+
+// launch(partial) writes n_blocks partial sums; they are summed in a fixed order on the device
+// and the total is returned (the stream is synchronised).
+template <class F>
+double sum_prob_partials(u64 n_blocks, F &&launch) {
+    Runtime &rt = runtime();
+    hipStream_t s = stream();
+    const size_t bytes = sizeof(double) * (size_t)(n_blocks + 1);
+    double *partial = static_cast<double *>(dev_alloc(bytes));
+    launch(partial);
+    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, partial, n_blocks, partial + n_blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rt.h_pinned, partial + n_blocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(partial, bytes);
+    QG_HIP(e);
+    return rt.h_pinned[0];
+}
+
+}  // namespace
+
 template <class R>
 double calc_prob(const Cx<R> *a, int n_lanes, int lane) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
     constexpr int W = Unit<R>::W, lw = lw_of(W);
-    Runtime &rt = runtime();
     hipStream_t s = stream();
     const u64 n_units_total = (u64(1) << n_lanes) / W;
     BitInsert ins;
@@ -368,20 +469,30 @@ double calc_prob(const Cx<R> *a, int n_lanes, int lane) {
         amp_mask = 0;
         n_units = n_units_total / 2;
     }
-    const u64 per_block = (u64)kBlock * kProbUnitsPerThread;
-    const u64 n_blocks = (n_units + per_block - 1) / per_block;
-    const size_t bytes = sizeof(double) * (size_t)(n_blocks + 1);
-    double *partial = static_cast<double *>(dev_alloc(bytes));
-    hipLaunchKernelGGL((k_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, ins, amp_mask, n_units,
-                       partial);
-    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, partial, n_blocks, partial + n_blocks);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(rt.h_pinned, partial + n_blocks, sizeof(double), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dev_free(partial, bytes);
-    QG_HIP(e);
-    return rt.h_pinned[0];
+    const u64 n_blocks = prob_blocks(n_units);
+    return sum_prob_partials(n_blocks, [&](double *partial) {
+        hipLaunchKernelGGL((k_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, ins, amp_mask, n_units,
+                           partial);
+    });
+}
+
+template <class R>
+double decohere_prob(Cx<R> *a, int n_lanes, int lane, int value, R norm, int prob_lane) {
+    QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
+    QG_REQUIRE(0 <= prob_lane && prob_lane < n_lanes, "lane %d out of range", prob_lane);
+    constexpr int W = Unit<R>::W, lw = lw_of(W);
+    const u64 n_units = (u64(1) << n_lanes) / W;
+    const u64 dist = lane < lw ? 0 : u64(1) << (lane - lw);
+    const bool per_unit = dist < (u64)kWave;   // as launch_decohere
+    const BitInsert ins = per_unit ? no_bits() : one_bit(lane - lw);
+    const u64 n_items = per_unit ? n_units : n_units / 2;
+    const u64 p_unit = prob_lane < lw ? 0 : u64(1) << (prob_lane - lw);
+    const int p_amp = prob_lane < lw ? 1 << prob_lane : 0;
+    const u64 n_blocks = prob_blocks(n_items);
+    return sum_prob_partials(n_blocks, [&](double *partial) {
+        hipLaunchKernelGGL((k_decohere_prob<R>), (unsigned)n_blocks, kBlock, 0, stream(), a, ins, dist, value,
+                           norm, n_items, p_unit, p_amp, partial);
+    });
 }
 
 template <class R>
@@ -390,9 +501,12 @@ void launch_decohere(Cx<R> *a, int n_lanes, int lane, int value, R norm) {
     constexpr int W = Unit<R>::W, lw = lw_of(W);
     const u64 n_units = (u64(1) << n_lanes) / W;
     hipStream_t s = stream();
-    const BitInsert ins = lane < lw ? no_bits() : one_bit(lane - lw);
     const u64 dist = lane < lw ? 0 : u64(1) << (lane - lw);
-    const u64 n_items = lane < lw ? n_units : n_units / 2;
+    // one unit per thread for a lane inside the unit or inside a wavefront's units, else one
+    // (kept, dropped) pair per thread
+    const bool per_unit = dist < (u64)kWave;
+    const BitInsert ins = per_unit ? no_bits() : one_bit(lane - lw);
+    const u64 n_items = per_unit ? n_units : n_units / 2;
     for_each_piece(n_items, [&](u64 first, u64 count) {
         hipLaunchKernelGGL((k_decohere<R>), grid_for(count, kWave), kWave, 0, s, a, ins, dist, value, norm,
                            n_items, first);
@@ -543,6 +657,7 @@ void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, doubl
     template void launch_pauli_decohere<R>(Cx<R> *, int, u64, u64, int, double);              \
     template void launch_set_zero_state<R>(Cx<R> *, int);                                     \
     template double calc_prob<R>(const Cx<R> *, int, int);                                    \
+    template double decohere_prob<R>(Cx<R> *, int, int, int, R, int);                         \
     template void launch_decohere<R>(Cx<R> *, int, int, int, R);                              \
     template void launch_decohere_and_separate<R>(Cx<R> *, const Cx<R> *, int, int, int, R);  \
     template void launch_reset<R>(Cx<R> *, int, int);                                         \

@@ -81,6 +81,8 @@ template <class R> void launch_set_zero_state(Cx<R> *a, int n_lanes);
 // P(bit `lane` == 0), FP64 accumulation; synchronises.
 template <class R> double calc_prob(const Cx<R> *a, int n_lanes, int lane);
 template <class R> void launch_decohere(Cx<R> *a, int n_lanes, int lane, int value, R norm);
+// launch_decohere, then calc_prob(prob_lane) of the collapsed state, in one pass; synchronises.
+template <class R> double decohere_prob(Cx<R> *a, int n_lanes, int lane, int value, R norm, int prob_lane);
 // Pauli string P over lanes x | z (X: x, Z: z, Y: both): P(measuring P gives +1) in FP64,
 // i.e. sum over pairs of |((1 + P) psi)_y|^2 / 2; synchronises.
 template <class R> double calc_pauli_prob(const Cx<R> *a, int n_lanes, u64 x, u64 z);

@@ -203,3 +203,35 @@ def test_random_unitary_round_trip_24q(hip_module):
         st = sim.qubits.states[:]
         assert abs(st[0] - 1.) < (1e-11 if dtype == np.float64 else 5e-5)
         assert np.max(np.abs(st[1:])) < (1e-11 if dtype == np.float64 else 5e-5)
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
+def test_sampling_pool_sparse_states(hip_module, dtype):
+    """Pools of 2^24 bins (4096 tiles of 4096 bins in 4 super-tiles of 2^22 bins) over states
+    whose nonzero bins sit at tile and super-tile edges. Every draw, including u = 0 and
+    u = 1 - 2^-53, returns a nonzero bin: the first one above u on the CDF, or the last nonzero
+    bin when rounding leaves the CDF's end just below u (the zero bins after it are never
+    returned). With two bins of probability 1/2, draws below 1/2 return the first one."""
+    n = 24
+    q = new_qregs(n)
+    u = np.concatenate([[0., 5e-324, 1e-300, 0.25, 0.5 - 1e-5, 0.5 + 1e-5, 0.75, 1. - 2. ** -52, 1. - 2. ** -53],
+                        np.random.RandomState(5).random_sample(4096)])
+
+    def draws(ops):
+        sim = _sim(hip_module, ops, 'one_static', dtype, q)
+        nz = np.flatnonzero(np.asarray(sim.qubits.prob[:]) > 0.25)
+        obs = np.asarray(sim.qubits.create_sampling_pool(q).sample(len(u), u).intarray)
+        sim.terminate()
+        return nz, obs
+
+    for k in (0, 4095, 4096, (1 << 22) - 1, (1 << 22) + 7, (1 << 24) - 1):
+        nz, obs = draws([X(q[b]) for b in range(n) if (k >> b) & 1])
+        assert len(nz) == 1, (k, nz)
+        assert np.all(obs == nz[0]), (k, nz, np.unique(obs))
+    k0 = 4095
+    nz, obs = draws([X(q[b]) for b in range(12)] + [H(q[22])])
+    assert len(nz) == 2 and k0 in nz, nz
+    away = np.abs(u - 0.5) > 1e-6
+    want = np.where(u < 0.5, nz[0], nz[1])
+    assert np.all(obs[away] == want[away]), np.unique(obs[away] - want[away])
+    assert np.all(np.isin(obs, nz))
