@@ -79,6 +79,21 @@ __global__ __launch_bounds__(kBlock) void k_prob_partial(const Cx<R> *__restrict
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// out[k] = sum of partial[k * kSumRun ...] over one run of kSumRun partials, in a fixed order.
+constexpr int kSumRun = 8192;
+
+__global__ __launch_bounds__(1024) void k_sum_runs(const double *partial, u64 n, double *out) {
+    const u64 base = (u64)blockIdx.x * kSumRun;
+    double s = 0.;
+#pragma unroll
+    for (int r = 0; r < kSumRun / 1024; ++r) {
+        const u64 i = base + (u64)r * 1024 + threadIdx.x;
+        if (i < n) s += partial[i];
+    }
+    s = block_sum<1024>(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(1024) void k_sum_partials(const double *partial, u64 n, double *out) {
     double s = 0.;
     for (u64 i = threadIdx.x; i < n; i += 1024) s += partial[i];
@@ -129,62 +144,52 @@ __global__ __launch_bounds__(kWave) void k_decohere(Cx<R> *__restrict__ a, BitIn
     store_unit(a, drop, z);
 }
 
-// k_decohere's collapse (same three forms, each thread taking kProbUnitsPerThread items) and,
-// on the collapsed amplitudes as stored, partial sums of |a|^2 over the amplitudes whose unit
-// index has no bit of `p_unit` set and whose in-unit index has no bit of `p_amp` set: the
-// probability of the next lane, in the same pass.
+// k_decohere's collapse (same three forms, one item per thread) and, on the collapsed
+// amplitudes as stored, the sum per wavefront of |a|^2 over the amplitudes whose unit index has
+// no bit of `p_unit` set and whose in-unit index has no bit of `p_amp` set: the probability of
+// the next lane, in the same pass. One item per thread in single-wavefront blocks with no
+// barrier keeps the collapse's own speed (a 4 GiB collapse: 1.02 ms, against 1.18 ms with 8
+// items per thread in 256-thread blocks; the FP64 sum itself costs nothing measurable).
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_decohere_prob(Cx<R> *__restrict__ a, BitInsert ins, u64 dist,
-                                                          int value, R norm, u64 n_items, u64 p_unit,
-                                                          int p_amp, double *partial) {
+__global__ __launch_bounds__(kWave) void k_decohere_prob(Cx<R> *__restrict__ a, BitInsert ins, u64 dist,
+                                                         int value, R norm, u64 n_items, u64 first, u64 p_unit,
+                                                         int p_amp, double *partial) {
+    const u64 w = first + (u64)blockIdx.x * kWave + threadIdx.x;
     double s = 0.;
-    const u64 base = (u64)blockIdx.x * (kBlock * kProbUnitsPerThread) + threadIdx.x;
-    auto add = [&](u64 unit, const Amps<R> &v) {
-        if ((unit & p_unit) != 0) return;
-#pragma unroll
-        for (int j = 0; j < Unit<R>::W; ++j)
-            if ((j & p_amp) == 0) s += abs2d(v.v[j]);
-    };
-#pragma unroll 4
-    for (int r = 0; r < kProbUnitsPerThread; ++r) {
-        const u64 w = base + (u64)r * kBlock;
-        if (w >= n_items) break;
+    if (w < n_items) {   // no early return: the whole wavefront takes part in the sum
         Amps<R> v;
-        if (dist == 0) {
-            v = load_unit(a, w);
-#pragma unroll
-            for (int j = 0; j < Unit<R>::W; ++j)
-                v.v[j] = (j == value) ? cx<R>(v.v[j].re * norm, v.v[j].im * norm) : cx<R>(0, 0);
-            store_unit(a, w, v);
-            add(w, v);
-        } else if (ins.n == 0) {
-            if (((w & dist) != 0) == (value != 0)) {
-                v = load_unit(a, w);
-#pragma unroll
-                for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
-                add(w, v);
-            } else {
-#pragma unroll
-                for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(0, 0);
-            }
-            store_unit(a, w, v);
-        } else {
+        u64 unit = w;
+        bool kept = true;
+        if (dist != 0 && ins.n != 0) {   // the pair (u0, u0 | dist)
             const u64 u0 = insert_zero_bits(w, ins);
-            const u64 keep = value ? (u0 | dist) : u0, drop = value ? u0 : (u0 | dist);
+            unit = value ? (u0 | dist) : u0;
             Amps<R> z;
-            v = load_unit(a, keep);
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j) z.v[j] = cx<R>(0, 0);
+            store_unit(a, value ? u0 : (u0 | dist), z);
+        } else if (dist != 0) {          // unit w, kept or cleared
+            kept = ((w & dist) != 0) == (value != 0);
+        }
+        if (kept) {
+            v = load_unit(a, unit);
 #pragma unroll
             for (int j = 0; j < Unit<R>::W; ++j) {
-                v.v[j] = cx<R>(v.v[j].re * norm, v.v[j].im * norm);
-                z.v[j] = cx<R>(0, 0);
+                const bool on = dist != 0 || j == value;
+                v.v[j] = on ? cx<R>(v.v[j].re * norm, v.v[j].im * norm) : cx<R>(0, 0);
             }
-            store_unit(a, keep, v);
-            store_unit(a, drop, z);
-            add(keep, v);
+            if ((unit & p_unit) == 0) {
+#pragma unroll
+                for (int j = 0; j < Unit<R>::W; ++j)
+                    if ((j & p_amp) == 0) s += abs2d(v.v[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < Unit<R>::W; ++j) v.v[j] = cx<R>(0, 0);
         }
+        store_unit(a, unit, v);
     }
-    s = block_sum<kBlock>(s);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    s = wave_sum(s);
+    if (threadIdx.x == 0) partial[first / kWave + blockIdx.x] = s;
 }
 
 // dst[k] = norm * src[k with a `value` bit inserted at `lane`].
@@ -430,19 +435,28 @@ u64 prob_blocks(u64 n_items) {
     return (n_items + per_block - 1) / per_block;
 }
 
-// launch(partial) writes n_blocks partial sums; they are summed in a fixed order on the device
-// and the total is returned (the stream is synchronised).
+// launch(partial) writes n_partials partial sums; they are summed in a fixed order on the
+// device (runs of kSumRun first when there are more than one run's worth) and the total is
+// returned (the stream is synchronised).
 template <class F>
-double sum_prob_partials(u64 n_blocks, F &&launch) {
+double sum_prob_partials(u64 n_partials, F &&launch) {
     Runtime &rt = runtime();
     hipStream_t s = stream();
-    const size_t bytes = sizeof(double) * (size_t)(n_blocks + 1);
+    const u64 n_runs = n_partials > (u64)kSumRun ? (n_partials + kSumRun - 1) / kSumRun : 0;
+    const size_t bytes = sizeof(double) * (size_t)(n_partials + n_runs + 1);
     double *partial = static_cast<double *>(dev_alloc(bytes));
     launch(partial);
-    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, partial, n_blocks, partial + n_blocks);
+    const double *last = partial;
+    u64 n_last = n_partials;
+    if (n_runs > 0) {
+        hipLaunchKernelGGL(k_sum_runs, (unsigned)n_runs, 1024, 0, s, partial, n_partials, partial + n_partials);
+        last = partial + n_partials;
+        n_last = n_runs;
+    }
+    double *total = partial + n_partials + n_runs;
+    hipLaunchKernelGGL(k_sum_partials, 1, 1024, 0, s, last, n_last, total);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(rt.h_pinned, partial + n_blocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(rt.h_pinned, total, sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dev_free(partial, bytes);
     QG_HIP(e);
@@ -488,10 +502,13 @@ double decohere_prob(Cx<R> *a, int n_lanes, int lane, int value, R norm, int pro
     const u64 n_items = per_unit ? n_units : n_units / 2;
     const u64 p_unit = prob_lane < lw ? 0 : u64(1) << (prob_lane - lw);
     const int p_amp = prob_lane < lw ? 1 << prob_lane : 0;
-    const u64 n_blocks = prob_blocks(n_items);
-    return sum_prob_partials(n_blocks, [&](double *partial) {
-        hipLaunchKernelGGL((k_decohere_prob<R>), (unsigned)n_blocks, kBlock, 0, stream(), a, ins, dist, value,
-                           norm, n_items, p_unit, p_amp, partial);
+    // one partial per wavefront; pieces are powers of two of at least 2^12 items, so each
+    // piece starts on a wavefront boundary
+    return sum_prob_partials((n_items + kWave - 1) / kWave, [&](double *partial) {
+        for_each_piece(n_items, [&](u64 first, u64 count) {
+            hipLaunchKernelGGL((k_decohere_prob<R>), grid_for(count, kWave), kWave, 0, stream(), a, ins, dist,
+                               value, norm, n_items, first, p_unit, p_amp, partial);
+        });
     });
 }
 
