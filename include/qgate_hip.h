@@ -110,7 +110,10 @@ int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit
 int qghip_qubit_processor_join(qghip_qubit_processor *p, qghip_qubit_states *qs,
                                qghip_qubit_states *const *qstates_list, int n_qstates,
                                int n_new_lanes);
-/* qubit_processor_decohere (glue.cpp:337-348): collapse local_lane to `value` in place. */
+/* qubit_processor_decohere (glue.cpp:337-348): collapse local_lane to `value` in place. The
+ * collapse is recorded and launched by the next call that reads or writes qs (a following
+ * calc_probability runs it in the same pass as its sum); results are those of collapsing at
+ * once, and a device failure is reported by that next call. */
 int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double prob,
                                    qghip_qubit_states *qs, int local_lane);
 /* qubit_processor_decohere_and_separate (glue.cpp:351-365): collapse local_lane of qs to
