@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from qgate_amd.model import gate_type as gtype
+from qgate_amd.simulator import qubits
 from tests.helpers import tol
 from tests.test_dense_fusion import _prepared, _read
 
@@ -143,3 +144,39 @@ def test_recorded_collapse_dropped_with_its_state(hip_module):
     fresh.processor.synchronize()   # applies the queued work of every listed state vector
     assert abs(fresh.processor.calc_probability(fresh, 4) - p) < 5e-6
     fresh.delete()
+
+
+@pytest.mark.parametrize('n,dtype', [(28, np.float32), (27, np.float64)])
+def test_measure_all_full_size(hip_module, n, dtype):
+    """Every qubit of a 2 GiB random state measured in a shuffled order (each probability after
+    the first from the fused pass): afterwards every P(lane = 0) is 1 or 0 as the outcome says,
+    and the basis state the outcomes spell holds the whole norm (size-independent checks)."""
+    from qgate_amd.simulator.lanes import Lane
+    rs = np.random.RandomState(n)
+    qs = hip_module.create_qubit_states(dtype)
+    proc = qs.processor
+    proc.initialize_qubit_states(qs, n)
+    proc.reset_qubit_states(qs)
+    for lane in range(n):
+        proc.apply_gate(gtype.U(*rs.uniform(0, 6.3, 3)), False, qs, lane)
+    for lane in range(n - 1):
+        proc.apply_controlled_gate(gtype.X(), False, qs, [lane], lane + 1)
+    index = 0
+    for lane in rs.permutation(n):
+        p = proc.calc_probability(qs, int(lane))
+        v = 0 if rs.random_sample() < p else 1
+        proc.decohere(v, p, qs, int(lane))
+        index |= v << int(lane)
+    bar = 5e-6 if dtype == np.float32 else 1e-12
+    for lane in range(n):
+        want = 0. if (index >> lane) & 1 else 1.
+        assert abs(proc.calc_probability(qs, lane) - want) < bar, lane
+    lanes = []
+    for i in range(n):
+        ln = Lane(None, i)
+        ln.set_external(i)
+        lanes.append(ln)
+    out = np.zeros(1, np.complex128 if dtype == np.float64 else np.complex64)
+    hip_module.create_qubits_states_getter(dtype).get_states(out, 0, qubits.null, [(qs, lanes)], [], 1, index, 1)
+    assert abs(abs(complex(out[0])) - 1.) < bar, out
+    qs.delete()
