@@ -201,6 +201,11 @@ constexpr int kChunk = 16;
 constexpr int kTileChunks = 256;
 constexpr int kTile = kChunk * kTileChunks;   // 4096 bins
 constexpr int kSuperTiles = 1024;             // 2^22 bins per super-tile
+// Threads per tile block: 8 bins each. With the tile's 37 KB of LDS four blocks fit on a CU,
+// so 512 threads give twice the wavefronts of 256 (the first 256 threads sum the chunks).
+constexpr int kPoolThreads = 512;
+constexpr int kPoolRows = kTile / kPoolThreads;
+static_assert(kPoolThreads >= kTileChunks, "one thread per chunk for the chunk sums");
 
 struct StateSrc {};
 struct ProbSrc {};
@@ -208,51 +213,53 @@ struct ProbSrc {};
 // One block per tile: p = |a|^2 (from the state) or p given; chunk sums; serial scan of the
 // 256 chunk sums; tile total; last bin with p > 0.
 template <class R, bool FROM_STATE>
-__global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
+__global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
                                                       double *__restrict__ p, double *__restrict__ e_inc,
                                                       double *__restrict__ t_tot, i64 *__restrict__ t_last) {
     __shared__ double sp[kTile + kTile / kChunk];   // one pad slot per chunk
     __shared__ double sc[kTileChunks];
-    __shared__ i64 sl[kBlock / 64];
+    __shared__ i64 sl[kPoolThreads / 64];
     const i64 base = (i64)blockIdx.x * kTile;
-    // every load of the thread first (kChunk of them in flight), then the FP64 probabilities
-    double v[kChunk];
+    // every load of the thread first (kPoolRows of them in flight), then the FP64 probabilities
+    double v[kPoolRows];
     if constexpr (FROM_STATE) {
-        Cx<R> x[kChunk];
+        Cx<R> x[kPoolRows];
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) {
-            const i64 bin = base + r * kBlock + threadIdx.x;
+        for (int r = 0; r < kPoolRows; ++r) {
+            const i64 bin = base + r * kPoolThreads + threadIdx.x;
             x[r] = bin < n_bins ? a[bin] : cx<R>(0, 0);
         }
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) v[r] = abs2d(x[r]);
+        for (int r = 0; r < kPoolRows; ++r) v[r] = abs2d(x[r]);
     } else {
 #pragma unroll
-        for (int r = 0; r < kChunk; ++r) {
-            const i64 bin = base + r * kBlock + threadIdx.x;
+        for (int r = 0; r < kPoolRows; ++r) {
+            const i64 bin = base + r * kPoolThreads + threadIdx.x;
             v[r] = bin < n_bins ? p[bin] : 0.;
         }
     }
 #pragma unroll
-    for (int r = 0; r < kChunk; ++r) {
-        const int li = r * kBlock + threadIdx.x;
+    for (int r = 0; r < kPoolRows; ++r) {
+        const int li = r * kPoolThreads + threadIdx.x;
         const i64 bin = base + li;
         // probabilities from the state, and zeros for the padding up to whole tiles
         if (FROM_STATE || bin >= n_bins) p[bin] = v[r];
         sp[li + li / kChunk] = v[r];
     }
     __syncthreads();
-    // serial sum over this thread's chunk
+    // serial sum over this thread's chunk (threads below kTileChunks)
     const int c = threadIdx.x;
-    double s = 0.;
     i64 last = -1;
-    for (int k = 0; k < kChunk; ++k) {
-        const int li = c * kChunk + k;
-        const double x = sp[li + li / kChunk];
-        s = (k == 0) ? x : s + x;
-        if (x > 0.) last = base + li;
+    if (c < kTileChunks) {
+        double s = 0.;
+        for (int k = 0; k < kChunk; ++k) {
+            const int li = c * kChunk + k;
+            const double x = sp[li + li / kChunk];
+            s = (k == 0) ? x : s + x;
+            if (x > 0.) last = base + li;
+        }
+        sc[c] = s;
     }
-    sc[c] = s;
     // the tile's last nonzero bin: maximum over the wavefront, then over the waves (below)
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
@@ -281,12 +288,12 @@ __global__ __launch_bounds__(kBlock) void k_pool_tile(const Cx<R> *__restrict__ 
         if (l == 0) {
             i64 m = sl[0];
 #pragma unroll
-            for (int w = 1; w < kBlock / 64; ++w) m = max(m, sl[w]);
+            for (int w = 1; w < kPoolThreads / 64; ++w) m = max(m, sl[w]);
             t_last[blockIdx.x] = m;
         }
     }
     __syncthreads();
-    e_inc[(i64)blockIdx.x * kTileChunks + c] = sc[c];
+    if (c < kTileChunks) e_inc[(i64)blockIdx.x * kTileChunks + c] = sc[c];
 }
 
 // One wave per super-tile: serial inclusive scan of the tile totals in place; super total out.
@@ -419,7 +426,7 @@ template <class R, bool FROM_STATE>
 void pool_build(PoolData &pd, const Cx<R> *a) {
     hipStream_t s = stream();
     double *d_total = pd.h_inc + pd.n_supers + 1;
-    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE>), (unsigned)pd.n_tiles, kBlock, 0, s, a, pd.n,
+    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE>), (unsigned)pd.n_tiles, kPoolThreads, 0, s, a, pd.n,
                        pd.p, pd.e_inc, pd.m_inc, pd.t_last);
     QG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc, pd.t_last,
