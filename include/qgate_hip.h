@@ -47,8 +47,8 @@ const char *qghip_last_error(void);
 /* Select the GPU and create the library stream; replaces devices_initialize
  * (cudaext.cpp:32-58, called from cudaruntime.py:74-76 module_init). One device only. */
 int qghip_devices_initialize(int device_id);
-/* Free cached device memory and the stream; replaces devices_clear (cudaext.cpp:60-68,
- * cudaruntime.py:78-84 module_finalize). */
+/* Apply queued work (fused gates, recorded collapses), then free cached device memory and the
+ * stream; replaces devices_clear (cudaext.cpp:60-68, cudaruntime.py:78-84 module_finalize). */
 int qghip_devices_clear(void);
 /* Device id, compute-unit count and HBM size of the initialised device. */
 int qghip_device_info(int *device_id, int *n_cus, uint64_t *total_mem);

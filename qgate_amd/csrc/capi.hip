@@ -246,7 +246,10 @@ int qghip_devices_initialize(int device_id) {
 }
 
 int qghip_devices_clear(void) {
-    return guarded([&] { runtime_clear(); });
+    return guarded([&] {
+        flush_all();   // queued gates and recorded collapses first, on the stream about to go
+        runtime_clear();
+    });
 }
 
 int qghip_set_launch_piece(int64_t items) {
