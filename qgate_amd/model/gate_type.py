@@ -49,10 +49,6 @@ def _u(theta, phi, lam):
             [cmath.exp(-0.5j * (lam - phi)) * s, cmath.exp(0.5j * (lam + phi)) * c]]
 
 
-def _u2(phi, lam):
-    return [[_R * v for v in row] for row in _u(math.pi / 2., phi, lam)]
-
-
 def _rx(theta):
     c, s = math.cos(theta / 2.), math.sin(theta / 2.)
     return [[c, -1j * s], [-1j * s, c]]

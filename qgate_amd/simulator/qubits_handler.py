@@ -36,7 +36,10 @@ class QubitsHandler:
         assert 1 < len(qregset), '2 or more qregs required.'
         by_qstates = {}
         new_qregs = []
-        for qreg in qregset:
+        # qregs in id order: state vectors of equal lane count keep the order of their
+        # smallest qreg id, so the local layout does not depend on set iteration order
+        # (which follows the qregs' hashes in the reference, qubits_handler.py:41-60)
+        for qreg in sorted(qregset, key=lambda q: q.id):
             if qreg in self.lanes:
                 lane = self.lanes[qreg]
                 by_qstates.setdefault(id(lane.qstates), (lane.qstates, []))[1].append(lane)

@@ -45,7 +45,7 @@ def _qft_random(n, gates, seed):
 
 
 # name: (circuit, prep, dtype, pool ordering (None: all qubits), draws, seed of the uniforms,
-#        measure every qubit at the end)
+#        measure at the end: True for every qubit, or a list of qubit indices, in that order)
 CASES = {
     'brickwork10_c128_dynamic': (_brickwork(10, 20, 1), 'dynamic', np.float64, None, 4096, 1001, False),
     'brickwork10_c128_static': (_brickwork(10, 20, 1), 'one_static', np.float64, None, 4096, 1001, False),
@@ -54,6 +54,15 @@ CASES = {
     'random12_c64': (_random(12, 300, 2), 'one_static', np.float32, None, 4096, 1002, True),
     'qft_random10_c128': (_qft_random(10, 50, 3), 'one_static', np.float64, None, 4096, 1003, False),
     'random11_c128_hidden': (_random(11, 200, 4), 'dynamic', np.float64, [9, 2, 5, 0, 7], 4096, 1004, False),
+    # dynamic preparation with measurements of some qubits: each Measure is followed by a
+    # Separate, so these pin the collapse into a smaller state vector at 0 < p < 1 (the
+    # unmeasured qubits keep the rescaled amplitudes)
+    'brickwork10_c128_dynamic_measure': (_brickwork(10, 20, 5), 'dynamic', np.float64, None, 4096, 1005,
+                                         [7, 2, 4, 9, 0]),
+    'brickwork10_c64_dynamic_measure': (_brickwork(10, 20, 5), 'dynamic', np.float32, None, 4096, 1005,
+                                        [7, 2, 4, 9, 0]),
+    'random11_c128_dynamic_measure': (_random(11, 150, 6), 'dynamic', np.float64, None, 4096, 1006,
+                                      [1, 10, 5, 6]),
 }
 
 
@@ -69,13 +78,14 @@ def run_case(module, name):
     u = np.random.RandomState(useed).random_sample(draws)
     out['uniforms'] = u
     out['samples'] = np.array(sim.qubits.create_sampling_pool(pool_q).sample(len(u), u).intarray, np.int64)
-    if measure_all:   # a second run of the same circuit followed by a measurement of every qubit
+    if measure_all:   # a second run of the same circuit followed by measurements
         q, ops = make()
-        refs = new_references(len(q))
+        which = list(range(len(q))) if measure_all is True else list(measure_all)
+        refs = new_references(len(which))
         msim = Simulator(module, circuit_prep=prep, dtype=dtype)
         msim.qubits.set_ordering(list(q))
         np.random.seed(useed + 1)
-        msim.run(ops + [measure(refs[i], q[i]) for i in range(len(q))])
+        msim.run(ops + [measure(refs[k], q[i]) for k, i in enumerate(which)])
         out['outcomes'] = np.array(msim.values.get(refs), np.int64)
         out['states_after_measure'] = np.array(msim.qubits.states[:])
         msim.terminate()
