@@ -85,6 +85,10 @@ class PauliSum:
         self.xmask = np.ascontiguousarray(xm, np.uint64)
         self.zmask = np.ascontiguousarray(zm, np.uint64)
 
+    def key(self):
+        """Value identity of the observable (coefficients and masks)."""
+        return (self.coefs.tobytes(), self.xmask.tobytes(), self.zmask.tobytes())
+
     @classmethod
     def zz_x(cls, a, b):
         """sum_i a_i Z_i Z_{i+1} + sum_i b_i X_i (open chain)."""
@@ -123,6 +127,7 @@ class ParametricCircuit:
         self.ptr = h.value
         self.ops = []
         self.n_params = 0
+        self._last = None   # (params, observable key) of the forward pass the library holds
 
     def __del__(self):
         self.delete()
@@ -188,12 +193,15 @@ class ParametricCircuit:
     def expectation(self, params, observable):
         p = self._params(params)
         out = ctypes.c_double()
+        self._last = None
         hipruntime.check(_lib().qghip_circuit_expectation(
             self.ptr, _dptr(p), len(p), len(observable.coefs), _dptr(observable.coefs),
             _uptr(observable.xmask), _uptr(observable.zmask), ctypes.byref(out)))
+        self._last = (p.tobytes(), observable.key())
         return out.value
 
     def backward(self, params):
+        """Gradient for the observable of the last expectation() call."""
         p = self._params(params)
         grad = np.zeros(max(1, self.n_params), np.float64)
         hipruntime.check(_lib().qghip_circuit_backward(self.ptr, _dptr(p), len(p), _dptr(grad)))
@@ -203,10 +211,21 @@ class ParametricCircuit:
         p = self._params(params)
         out = ctypes.c_double()
         grad = np.zeros(max(1, self.n_params), np.float64)
+        self._last = None
         hipruntime.check(_lib().qghip_circuit_gradient(
             self.ptr, _dptr(p), len(p), len(observable.coefs), _dptr(observable.coefs),
             _uptr(observable.xmask), _uptr(observable.zmask), ctypes.byref(out), _dptr(grad)))
+        self._last = (p.tobytes(), observable.key())
         return out.value, grad[:self.n_params]
+
+    def gradient_for(self, params, observable):
+        """d<H>/dparams for `observable`: the backward sweep alone when the library still
+        holds the forward pass of these parameters and this observable, else forward and
+        backward."""
+        p = self._params(params)
+        if self._last == (p.tobytes(), observable.key()):
+            return self.backward(p)
+        return self.gradient(p, observable)[1]
 
     def state(self):
         """Final state of the last expectation call (qubit q = bit q)."""
@@ -244,13 +263,17 @@ try:
             p = params.detach().to('cpu', torch.float64).numpy()
             value = circuit.expectation(p, observable)
             ctx.circuit = circuit
+            ctx.observable = observable
             ctx.params = p.copy()
             ctx.param_dtype, ctx.param_device = params.dtype, params.device
             return torch.tensor(value, dtype=params.dtype, device=params.device)
 
         @staticmethod
         def backward(ctx, grad_output):
-            g = ctx.circuit.backward(ctx.params)
+            # the observable of this forward: another expectation call on the same circuit
+            # between forward and backward (or a second observable in the same loss) re-runs
+            # the forward pass for this one
+            g = ctx.circuit.gradient_for(ctx.params, ctx.observable)
             grad = torch.as_tensor(g, dtype=ctx.param_dtype).to(ctx.param_device)
             return grad_output * grad, None, None
 except ImportError:  # torch is only needed for the autograd wrapper

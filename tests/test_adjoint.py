@@ -247,6 +247,32 @@ def test_torch_autograd(hip_module):
 
 
 @pytest.mark.gpu
+def test_torch_autograd_two_observables(hip_module, port_module):
+    """A loss that sums two observables of one circuit: each backward uses its own
+    forward's observable, checked against the oracle's parameter-shift gradients."""
+    import torch
+    from qgate_amd.adjoint import Expectation
+    spec = ac.hea_spec(5, 2)
+    h1 = PauliSum.zz_x(np.linspace(-1., 1., 4), np.zeros(5))
+    h2 = PauliSum([(0.7, {0: 'X', 2: 'Y'}), (-1.3, {4: 'Z'}), (0.4, {1: 'Y'})])
+    circ = spec.build(np.float64)
+    params = np.random.RandomState(4).uniform(-1, 1, spec.n_params)
+    t = torch.tensor(params, dtype=torch.float64, requires_grad=True)
+    loss = 2. * Expectation.apply(t, circ, h1) - 0.5 * Expectation.apply(t, circ, h2)
+    loss.backward()
+    g1 = ac.parameter_shift_gradient(port_module, spec, params, h1)
+    g2 = ac.parameter_shift_gradient(port_module, spec, params, h2)
+    err = float(np.max(np.abs(t.grad.numpy() - (2. * g1 - 0.5 * g2))))
+    assert err < 1e-11, err
+    # an expectation call between forward and backward does not change the gradient
+    t.grad = None
+    y = Expectation.apply(t, circ, h1)
+    circ.expectation(params + 0.3, h2)
+    y.backward()
+    assert float(np.max(np.abs(t.grad.numpy() - g1))) < 1e-11
+
+
+@pytest.mark.gpu
 def test_argument_errors(hip_module):
     from qgate_amd.adjoint import ParametricCircuit
     circ = ParametricCircuit(4, np.float64)
