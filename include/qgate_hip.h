@@ -105,6 +105,19 @@ int qghip_qubit_processor_reset_qubit_states(qghip_qubit_processor *p, qghip_qub
 /* qubit_processor_calc_probability (glue.cpp:302-311): P(bit local_lane == 0), FP64. */
 int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit_states *qs,
                                            int local_lane, double *prob);
+/* Addition, no reference counterpart: qubit_processor_calc_probability for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and summation order, so the same bits as
+ * qghip_qubit_processor_calc_probability on a state with no recorded collapse; the partial
+ * sums go to a fixed 32 MiB device array the library holds per GPU for the life of the
+ * process, so states of at most 35 lanes are accepted (QGHIP_EVALUE beyond). A recorded
+ * collapse is applied first in its own pass (the result then differs from the fused path of
+ * calc_probability by FP64 rounding only); queued fused gates are refused (QGHIP_EVALUE:
+ * synchronize first). Every call on a GPU shares that GPU's array; the library orders its
+ * calls on one stream under one lock, and work that overlapped a call on the array would
+ * race. Calls no allocator. */
+int qghip_qubit_processor_calc_probability_noalloc(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                   int local_lane, double *prob);
 /* qubit_processor_join (glue.cpp:314-334): qs <- kron(list[0], ..., list[n-1]) with the last
  * entry in the lowest bits; the n_new_lanes top lanes start in |0>. */
 int qghip_qubit_processor_join(qghip_qubit_processor *p, qghip_qubit_states *qs,

@@ -48,6 +48,19 @@ struct qghip_sampling_pool {
 
 namespace {
 
+// Partial sums of qghip_qubit_processor_calc_probability_noalloc: a file-scope device array
+// instead of an allocation. 2^22 block partials cover P(lane = 0) of every state of up to 35
+// lanes (k_prob_partial sums 4096 units per block); the run sums and the total follow them.
+// Each device holds its own copy at its own address for the life of the process. In host code
+// `scratch` names a host-side stand-in, never device memory: kernels and copies get the
+// address hipGetSymbolAddress returns for the current device (noalloc_scratch()).
+constexpr u64 kScratchPartials = u64(1) << 22;
+constexpr u64 kScratchDoubles = kScratchPartials + kScratchPartials / 8192 + 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
 struct ValueError : Error {
     using Error::Error;
 };
@@ -241,8 +254,34 @@ extern "C" {
 
 const char *qghip_last_error(void) { return last_error().c_str(); }
 
+namespace {
+
+// Device address of `scratch` on the runtime's device, looked up once per device with that
+// device current (runtime_init made it current; the library uses no other).
+double *noalloc_scratch() {
+    Runtime &rt = runtime();
+    stream();   // initialises the runtime on the default device if nothing did yet
+    QG_REQUIRE(0 <= rt.device && rt.device < kMaxScratchDevices, "device %d out of range", rt.device);
+    double *&ptr = g_scratch_ptr[rt.device];
+    if (ptr == nullptr) {
+        int cur = -1;
+        QG_HIP(hipGetDevice(&cur));
+        QG_REQUIRE(cur == rt.device, "device %d is not current", rt.device);
+        void *p = nullptr;
+        QG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QG_REQUIRE(p != nullptr, "no device address for the scratch array");
+        ptr = static_cast<double *>(p);
+    }
+    return ptr;
+}
+
+}  // namespace
+
 int qghip_devices_initialize(int device_id) {
-    return guarded([&] { runtime_init(device_id); });
+    return guarded([&] {
+        runtime_init(device_id);
+        noalloc_scratch();
+    });
 }
 
 int qghip_devices_clear(void) {
@@ -414,6 +453,28 @@ int qghip_qubit_processor_calc_probability(qghip_qubit_processor *p, qghip_qubit
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             *prob = calc_prob<R>(camps<R>(qs), qs->n_lanes, local_lane);
+        });
+    });
+}
+
+int qghip_qubit_processor_calc_probability_noalloc(qghip_qubit_processor *p, qghip_qubit_states *qs,
+                                                   int local_lane, double *prob) {
+    return guarded([&] {
+        if (p == nullptr || prob == nullptr) value_error("null argument");
+        check_qs(qs, p->prec);
+        check_lane(qs, local_lane);
+        // applying queued gates can allocate (dense blocks upload their matrices)
+        if (!qs->pending.empty()) value_error("gates are queued on this state vector; synchronize first");
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            if (calc_prob_buffer_len<R>(qs->n_lanes, local_lane) > kScratchDoubles)
+                value_error("state vector too large for the fixed scratch array (at most 35 lanes)");
+        });
+        double *buf = noalloc_scratch();
+        flush(qs);   // a recorded collapse only: k_decohere, no allocation
+        by_prec(p->prec, [&](auto r) {
+            typedef decltype(r) R;
+            *prob = calc_prob_in<R>(camps<R>(qs), qs->n_lanes, local_lane, buf, kScratchDoubles);
         });
     });
 }

@@ -435,16 +435,24 @@ u64 prob_blocks(u64 n_items) {
     return (n_items + per_block - 1) / per_block;
 }
 
+// Doubles the summation of n_partials partial sums needs: the partials, one sum per run of
+// kSumRun (when there is more than one run's worth) and the total.
+u64 sum_buffer_len(u64 n_partials) {
+    const u64 n_runs = n_partials > (u64)kSumRun ? (n_partials + kSumRun - 1) / kSumRun : 0;
+    return n_partials + n_runs + 1;
+}
+
 // launch(partial) writes n_partials partial sums; they are summed in a fixed order on the
 // device (runs of kSumRun first when there are more than one run's worth) and the total is
-// returned (the stream is synchronised).
+// returned (the stream is synchronised). The buffer is taken from the allocator, or is
+// `buf` (sum_buffer_len(n_partials) doubles, every one written before it is read) when given.
 template <class F>
-double sum_prob_partials(u64 n_partials, F &&launch) {
+double sum_prob_partials(u64 n_partials, F &&launch, double *buf = nullptr) {
     Runtime &rt = runtime();
     hipStream_t s = stream();
     const u64 n_runs = n_partials > (u64)kSumRun ? (n_partials + kSumRun - 1) / kSumRun : 0;
-    const size_t bytes = sizeof(double) * (size_t)(n_partials + n_runs + 1);
-    double *partial = static_cast<double *>(dev_alloc(bytes));
+    const size_t bytes = sizeof(double) * (size_t)sum_buffer_len(n_partials);
+    double *partial = buf != nullptr ? buf : static_cast<double *>(dev_alloc(bytes));
     launch(partial);
     const double *last = partial;
     u64 n_last = n_partials;
@@ -458,36 +466,63 @@ double sum_prob_partials(u64 n_partials, F &&launch) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(rt.h_pinned, total, sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dev_free(partial, bytes);
+    if (buf == nullptr) dev_free(partial, bytes);
     QG_HIP(e);
     return rt.h_pinned[0];
+}
+
+// Units k_prob_partial sums for P(lane = 0): the half with the lane bit clear (lane at unit
+// level), or every unit counting even amplitudes only (complex64 lane 0).
+template <class R>
+struct ProbRange {
+    BitInsert ins;
+    int amp_mask;
+    u64 n_units;
+    ProbRange(int n_lanes, int lane) {
+        constexpr int W = Unit<R>::W, lw = lw_of(W);
+        const u64 n_units_total = (u64(1) << n_lanes) / W;
+        if (lane < lw) {
+            ins = no_bits();
+            amp_mask = 1;
+            n_units = n_units_total;
+        } else {
+            ins = one_bit(lane - lw);
+            amp_mask = 0;
+            n_units = n_units_total / 2;
+        }
+    }
+};
+
+template <class R>
+double calc_prob_with(const Cx<R> *a, int n_lanes, int lane, double *buf) {
+    QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
+    hipStream_t s = stream();
+    const ProbRange<R> r(n_lanes, lane);
+    const u64 n_blocks = prob_blocks(r.n_units);
+    return sum_prob_partials(n_blocks, [&](double *partial) {
+        hipLaunchKernelGGL((k_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, r.ins, r.amp_mask, r.n_units,
+                           partial);
+    }, buf);
 }
 
 }  // namespace
 
 template <class R>
 double calc_prob(const Cx<R> *a, int n_lanes, int lane) {
+    return calc_prob_with<R>(a, n_lanes, lane, nullptr);
+}
+
+template <class R>
+u64 calc_prob_buffer_len(int n_lanes, int lane) {
     QG_REQUIRE(0 <= lane && lane < n_lanes, "lane %d out of range", lane);
-    constexpr int W = Unit<R>::W, lw = lw_of(W);
-    hipStream_t s = stream();
-    const u64 n_units_total = (u64(1) << n_lanes) / W;
-    BitInsert ins;
-    int amp_mask;
-    u64 n_units;
-    if (lane < lw) {   // complex64 lane 0: every unit, even amplitudes
-        ins = no_bits();
-        amp_mask = 1;
-        n_units = n_units_total;
-    } else {
-        ins = one_bit(lane - lw);
-        amp_mask = 0;
-        n_units = n_units_total / 2;
-    }
-    const u64 n_blocks = prob_blocks(n_units);
-    return sum_prob_partials(n_blocks, [&](double *partial) {
-        hipLaunchKernelGGL((k_prob_partial<R>), (unsigned)n_blocks, kBlock, 0, s, a, ins, amp_mask, n_units,
-                           partial);
-    });
+    return sum_buffer_len(prob_blocks(ProbRange<R>(n_lanes, lane).n_units));
+}
+
+template <class R>
+double calc_prob_in(const Cx<R> *a, int n_lanes, int lane, double *buf, u64 buf_len) {
+    QG_REQUIRE(buf != nullptr, "null partial-sum buffer");
+    QG_REQUIRE(calc_prob_buffer_len<R>(n_lanes, lane) <= buf_len, "partial-sum buffer too small");
+    return calc_prob_with<R>(a, n_lanes, lane, buf);
 }
 
 template <class R>
@@ -674,6 +709,8 @@ void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, doubl
     template void launch_pauli_decohere<R>(Cx<R> *, int, u64, u64, int, double);              \
     template void launch_set_zero_state<R>(Cx<R> *, int);                                     \
     template double calc_prob<R>(const Cx<R> *, int, int);                                    \
+    template u64 calc_prob_buffer_len<R>(int, int);                                           \
+    template double calc_prob_in<R>(const Cx<R> *, int, int, double *, u64);                  \
     template double decohere_prob<R>(Cx<R> *, int, int, int, R, int);                         \
     template void launch_decohere<R>(Cx<R> *, int, int, int, R);                              \
     template void launch_decohere_and_separate<R>(Cx<R> *, const Cx<R> *, int, int, int, R);  \

@@ -80,6 +80,11 @@ double pauli_expectation(const Cx<R> *psi, Cx<R> *lam, int n_lanes, const std::v
 template <class R> void launch_set_zero_state(Cx<R> *a, int n_lanes);
 // P(bit `lane` == 0), FP64 accumulation; synchronises.
 template <class R> double calc_prob(const Cx<R> *a, int n_lanes, int lane);
+// Doubles of partial-sum buffer calc_prob_in needs for this lane (the partials, run sums and
+// total), and calc_prob with that buffer given instead of taken from the allocator (no
+// allocation; the buffer must hold at least that many doubles).
+template <class R> u64 calc_prob_buffer_len(int n_lanes, int lane);
+template <class R> double calc_prob_in(const Cx<R> *a, int n_lanes, int lane, double *buf, u64 buf_len);
 template <class R> void launch_decohere(Cx<R> *a, int n_lanes, int lane, int value, R norm);
 // launch_decohere, then calc_prob(prob_lane) of the collapsed state, in one pass; synchronises.
 template <class R> double decohere_prob(Cx<R> *a, int n_lanes, int lane, int value, R norm, int prob_lane);

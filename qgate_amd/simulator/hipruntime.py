@@ -75,6 +75,7 @@ def lib():
         'qghip_qubit_processor_initialize_qubit_states': (ctypes.c_int, [H, H, ctypes.c_int]),
         'qghip_qubit_processor_reset_qubit_states': (ctypes.c_int, [H, H]),
         'qghip_qubit_processor_calc_probability': (ctypes.c_int, [H, H, ctypes.c_int, _c_double_p]),
+        'qghip_qubit_processor_calc_probability_noalloc': (ctypes.c_int, [H, H, ctypes.c_int, _c_double_p]),
         'qghip_qubit_processor_join': (ctypes.c_int, [H, H, _c_void_pp, ctypes.c_int, ctypes.c_int]),
         'qghip_qubit_processor_decohere': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double, H, ctypes.c_int]),
         'qghip_qubit_processor_decohere_and_separate': (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double, H, H, H, ctypes.c_int]),
