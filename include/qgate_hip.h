@@ -33,6 +33,7 @@ enum { QGHIP_OK = 0, QGHIP_ERROR = -1, QGHIP_EVALUE = -2 };
 enum { QGHIP_FP64 = 0, QGHIP_FP32 = 1 };
 enum { QGHIP_MATHOP_NULL = 0, QGHIP_MATHOP_PROB = 1 };
 enum { QGHIP_FUSION_OFF = 0, QGHIP_FUSION_TILE = 1, QGHIP_FUSION_DENSE = 2 };
+enum { QGHIP_POOL_AUTO = 0, QGHIP_POOL_BOUNDED = 1, QGHIP_POOL_FULL = 2 };
 
 typedef struct qghip_qubit_states qghip_qubit_states;
 typedef struct qghip_qubit_processor qghip_qubit_processor;
@@ -195,7 +196,11 @@ int qghip_qubits_states_getter_prepare_prob_array(qghip_qubits_states_getter *g,
                                                   qghip_qubit_states *const *qstates_list,
                                                   int n_qstates, int n_lanes, int n_hidden_lanes);
 /* qubits_states_getter_create_sampling_pool (glue.cpp:539-559): device-resident FP64 CDF of
- * the same marginal distribution; empty_lanes are external bit positions that read as 0. */
+ * the same marginal distribution; empty_lanes are external bit positions that read as 0.
+ * A pool over every lane of one state vector in its own order may be bounded (see
+ * qghip_set_sampling_pool_workspace): it then keeps only prefix sums per 16 bins (2^n / 16
+ * doubles) and reads |a|^2 of the found 16 bins from the state vector at draw time, so
+ * sampling after that state vector is written or freed fails with QGHIP_EVALUE. */
 int qghip_qubits_states_getter_create_sampling_pool(qghip_qubits_states_getter *g,
                                                     const int *lane_tables,
                                                     qghip_qubit_states *const *qstates_list,
@@ -213,6 +218,13 @@ int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations,
                                int64_t array_size, int64_t n_samples, const double *randnum);
 /* sampling_pool_delete (glue.cpp:592-602). */
 int qghip_sampling_pool_delete(qghip_sampling_pool *pool);
+/* Addition, no reference counterpart: workspace of the sampling pools created from now on.
+ * QGHIP_POOL_FULL keeps the FP64 probability of every bin (8 bytes per bin plus the prefix
+ * sums), as the reference's pools do (CPUSamplingPool.cpp:8-63); QGHIP_POOL_BOUNDED keeps
+ * only the prefix sums (1/16 of that) for pools over a whole state vector in its own lane
+ * order, and reads the amplitudes at draw time; QGHIP_POOL_AUTO (default) is FULL unless the
+ * probability array would not fit in free device memory. Draws are identical in all modes. */
+int qghip_set_sampling_pool_workspace(int mode);
 
 /* ---- adjoint differentiation (no reference counterpart) ------------------------------ */
 
@@ -301,6 +313,12 @@ int qghip_set_register_tiles(int on);
 int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const double *mat8,
                        const int *local_control_lanes, int n_controls, int local_target_lane,
                        int reps, double *avg_ms);
+
+/* Time `reps` runs of the library's non-temporal streaming copy (16-byte units, four loads
+ * in flight per thread) over two fresh device buffers of `bytes` bytes (a multiple of 16),
+ * HIP events on the library stream; avg_ms receives the mean kernel duration. The copy moves
+ * 2 * bytes per run: the HBM streaming ceiling the gate kernels are compared with. */
+int qghip_profile_copy(int64_t bytes, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

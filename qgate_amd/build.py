@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIBDIR = os.path.join(HERE, 'lib')
 LIBNAME = 'libqgate_hip.so'
 SOURCES = ['qg_runtime.hip', 'gate_matrix.hip', 'k_gates.hip', 'k_state.hip', 'k_getter.hip',
-           'capi.hip', 'adjoint.hip', 'k_tile.hip', 'k_batch.hip']
+           'capi.hip', 'adjoint.hip', 'k_tile.hip', 'k_batch.hip', 'k_marginal.hip']
 ARCH = os.environ.get('QGATE_HIP_ARCH', 'gfx950')
 
 # -packed-fp32-ops: no v_pk_*_f32 instructions are generated.

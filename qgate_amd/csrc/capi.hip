@@ -26,6 +26,8 @@ struct qghip_qubit_states {
     // the pending gates; a calc_probability that comes next runs it in the same pass.
     int dec_lane = -1, dec_value = 0;
     double dec_norm = 0.;
+    // bounded sampling pools that read this state vector when they draw
+    std::vector<qghip_sampling_pool *> readers;
 
     size_t n_states() const { return size_t(1) << n_lanes; }
 };
@@ -44,6 +46,9 @@ struct qghip_qubits_states_getter {
 struct qghip_sampling_pool {
     PoolData pool;
     std::vector<int> empty_lanes;
+    // bounded pool: the state vector it draws from; stale once that vector is written
+    qghip_qubit_states *src = nullptr;
+    bool stale = false;
 };
 
 namespace {
@@ -59,6 +64,9 @@ constexpr u64 kScratchDoubles = kScratchPartials + kScratchPartials / 8192 + 1;
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
 constexpr int kMaxScratchDevices = 64;
+
+// Sampling-pool workspace: QGHIP_POOL_AUTO, _BOUNDED or _FULL (qghip_set_sampling_pool_workspace).
+int g_pool_workspace = 0;
 double *g_scratch_ptr[kMaxScratchDevices] = {};
 
 struct ValueError : Error {
@@ -121,6 +129,16 @@ constexpr int kDenseWidth = 6;   // default block width of QGHIP_FUSION_DENSE
 
 bool queued(const qghip_qubit_states *qs) { return !qs->pending.empty() || qs->dec_lane >= 0; }
 
+// qs is about to be written or freed: bounded pools drawing from it can no longer draw.
+void touch(qghip_qubit_states *qs) {
+    if (qs == nullptr) return;
+    for (qghip_sampling_pool *pl : qs->readers) {
+        pl->stale = true;
+        pl->src = nullptr;
+    }
+    qs->readers.clear();
+}
+
 // Apply the queued work of qs: the recorded collapse, then the queued gates.
 void flush(qghip_qubit_states *qs) {
     if (qs == nullptr || !queued(qs)) return;
@@ -139,24 +157,36 @@ void flush(qghip_qubit_states *qs) {
                                     qs->dec_norm);
     }
     if (gates.empty()) return;
-    if (qs->fuse_mode == QGHIP_FUSION_DENSE) {
-        const int width = qs->tile_bits > 0 ? qs->tile_bits : kDenseWidth;
+    try {
+        if (qs->fuse_mode == QGHIP_FUSION_DENSE) {
+            const int width = qs->tile_bits > 0 ? qs->tile_bits : kDenseWidth;
+            if (qs->prec == precFP32)
+                apply_dense_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, width);
+            else
+                apply_dense_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, width);
+            return;
+        }
         if (qs->prec == precFP32)
-            apply_dense_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, width);
+            apply_gate_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
+                                       qs->low_bits);
         else
-            apply_dense_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, width);
-        return;
+            apply_gate_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
+                                        qs->low_bits);
+    } catch (...) {
+        // an argument the queue was built under is refused before any launch: keep the gates
+        // queued so that the failure is reported again, not lost (a failure after some passes
+        // ran cannot be undone; the error then reports it)
+        if (qs->pending.empty()) {
+            qs->pending.swap(gates);
+            pending_states().push_back(qs);
+        }
+        throw;
     }
-    if (qs->prec == precFP32)
-        apply_gate_sequence<float>(static_cast<Cx<float> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
-                                   qs->low_bits);
-    else
-        apply_gate_sequence<double>(static_cast<Cx<double> *>(qs->d_ptr), qs->n_lanes, gates, qs->tile_bits,
-                                    qs->low_bits);
 }
 
 // Forget the queued work of qs (its contents are about to be overwritten or freed).
 void drop(qghip_qubit_states *qs) {
+    touch(qs);
     if (qs == nullptr || !queued(qs)) return;
     qs->pending.clear();
     qs->dec_lane = -1;
@@ -209,6 +239,7 @@ void apply_matrix(const qghip_qubit_processor *p, Prec prec, qghip_qubit_states 
     std::sort(sorted.begin(), sorted.end());
     if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
         value_error("duplicate control lanes");
+    touch(qs);
     if (p != nullptr && (p->fusion == QGHIP_FUSION_TILE || p->fusion == QGHIP_FUSION_DENSE)) {
         // gates queued under other settings are applied under those first
         if (!qs->pending.empty() &&
@@ -299,6 +330,14 @@ int qghip_set_launch_piece(int64_t items) {
     });
 }
 
+int qghip_set_sampling_pool_workspace(int mode) {
+    return guarded([&] {
+        if (mode != QGHIP_POOL_AUTO && mode != QGHIP_POOL_BOUNDED && mode != QGHIP_POOL_FULL)
+            value_error("unknown sampling pool workspace mode");
+        g_pool_workspace = mode;
+    });
+}
+
 int qghip_set_register_tiles(int on) {
     return guarded([&] { set_register_tiles(on != 0); });
 }
@@ -385,6 +424,9 @@ int qghip_qubit_processor_set_fusion(qghip_qubit_processor *p, int mode, int til
         if (mode == QGHIP_FUSION_DENSE && (tile_bits == 1 || tile_bits > 6))
             value_error("dense block width must be 2..6");
         if (low_bits > 8) value_error("low_bits out of range");
+        // complex64 tiles move amplitudes in pairs, so their lowest bit stays in the tile
+        if (mode == QGHIP_FUSION_TILE && p->prec == precFP32 && low_bits == 0)
+            value_error("complex64 tile passes need low_bits >= 1");
         p->fusion = mode;
         p->tile_bits = tile_bits;
         p->low_bits = low_bits;
@@ -514,6 +556,7 @@ int qghip_qubit_processor_decohere(qghip_qubit_processor *p, int value, double p
         check_lane(qs, local_lane);
         if (value != 0 && value != 1) value_error("value must be 0 or 1");
         flush(qs);
+        touch(qs);
         // norm in FP64, rounded to the state precision (CPUQubitProcessor.cpp:227, :235). The
         // collapse is recorded and launched by the next call that touches qs (flush), or
         // together with the probability when that call is calc_probability.
@@ -571,6 +614,7 @@ int qghip_qubit_processor_pauli_decohere(qghip_qubit_processor *p, int value, do
         check_pauli_lanes(qs, x_lanes, z_lanes);
         if (value != 0 && value != 1) value_error("value must be 0 or 1");
         flush(qs);
+        touch(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             launch_pauli_decohere<R>(amps<R>(qs), qs->n_lanes, x_lanes, z_lanes, value, prob);
@@ -585,6 +629,7 @@ int qghip_qubit_processor_apply_reset(qghip_qubit_processor *p, qghip_qubit_stat
         check_qs(qs, p->prec);
         check_lane(qs, local_lane);
         flush(qs);
+        touch(qs);
         by_prec(p->prec, [&](auto r) {
             typedef decltype(r) R;
             launch_reset<R>(amps<R>(qs), qs->n_lanes, local_lane);
@@ -636,6 +681,7 @@ int qghip_qubit_processor_apply_unitary(qghip_qubit_processor *p, qghip_qubit_st
         std::sort(sorted.begin(), sorted.end());
         if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) value_error("duplicate unitary lanes");
         flush(qs);
+        touch(qs);
         const int D = 1 << k;
         std::vector<std::complex<double>> m(D * D);
         for (int i = 0; i < D * D; ++i) m[i] = std::complex<double>(mat[2 * i], mat[2 * i + 1]);
@@ -746,10 +792,26 @@ int qghip_qubits_states_getter_create_sampling_pool(qghip_qubits_states_getter *
         try {
             bool direct = n_hidden_lanes == 0 && views.size() == 1 && views[0].n_lanes == n_lanes;
             for (int l = 0; direct && l < n_lanes; ++l) direct = views[0].local_to_ext[l] == l;
+            bool bounded = false;
+            if (direct && g_pool_workspace == QGHIP_POOL_BOUNDED) {
+                bounded = true;
+            } else if (direct && g_pool_workspace == QGHIP_POOL_AUTO) {
+                // bounded when the probability array would not fit beside what is allocated
+                size_t free_b = 0, total_b = 0;
+                QG_HIP(hipMemGetInfo(&free_b, &total_b));
+                const size_t avail = free_b + dev_cached_bytes();
+                const size_t headroom = size_t(1) << 30;
+                bounded = pool_bytes(n_lanes, true) + headroom > avail;
+            }
             by_prec(g->prec, [&](auto r) {
                 typedef decltype(r) R;
                 if (direct) {
-                    pool_build_from_state<R>(pool->pool, static_cast<const Cx<R> *>(views[0].ptr), n_lanes);
+                    pool_build_from_state<R>(pool->pool, static_cast<const Cx<R> *>(views[0].ptr), n_lanes,
+                                             bounded);
+                    if (bounded) {
+                        pool->src = qstates_list[0];
+                        pool->src->readers.push_back(pool);
+                    }
                 } else {
                     pool_alloc(pool->pool, n_lanes);
                     try {
@@ -775,6 +837,9 @@ int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations,
         if (pool == nullptr || observations == nullptr || randnum == nullptr)
             value_error("null argument");
         if (n_samples < 0 || array_size < n_samples) value_error("array size too small.");
+        if (pool->stale)
+            value_error("the state vector this bounded sampling pool draws from has changed or been freed; "
+                        "create the pool again");
         pool_sample(pool->pool, reinterpret_cast<i64 *>(observations), n_samples, randnum,
                     pool->empty_lanes);
     });
@@ -783,6 +848,10 @@ int qghip_sampling_pool_sample(qghip_sampling_pool *pool, int64_t *observations,
 int qghip_sampling_pool_delete(qghip_sampling_pool *pool) {
     return guarded([&] {
         if (pool == nullptr) return;
+        if (pool->src != nullptr) {
+            auto &r = pool->src->readers;
+            r.erase(std::remove(r.begin(), r.end(), pool), r.end());
+        }
         pool_free(pool->pool);
         delete pool;
     });
@@ -810,6 +879,40 @@ int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const d
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         *avg_ms = ms / reps;
+    });
+}
+
+int qghip_profile_copy(int64_t bytes, int reps, double *avg_ms) {
+    return guarded([&] {
+        if (avg_ms == nullptr || reps < 1 || bytes < 16 || bytes % 16 != 0) value_error("bad argument");
+        void *src = dev_alloc((size_t)bytes), *dst = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            dst = dev_alloc((size_t)bytes);
+            hipStream_t s = stream();
+            QG_HIP(hipMemsetAsync(src, 0, (size_t)bytes, s));
+            launch_stream_copy(dst, src, (u64)bytes);   // warm-up
+            QG_HIP(hipEventCreate(&e0));
+            QG_HIP(hipEventCreate(&e1));
+            QG_HIP(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) launch_stream_copy(dst, src, (u64)bytes);
+            QG_HIP(hipEventRecord(e1, s));
+            QG_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            QG_HIP(hipEventElapsedTime(&ms, e0, e1));
+            *avg_ms = ms / reps;
+        } catch (...) {
+            sync();
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            dev_free(src, (size_t)bytes);
+            if (dst) dev_free(dst, (size_t)bytes);
+            throw;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        dev_free(src, (size_t)bytes);
+        dev_free(dst, (size_t)bytes);
     });
 }
 

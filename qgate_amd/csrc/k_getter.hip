@@ -3,7 +3,7 @@
 //
 // Reference semantics:
 //   get_states        qgate/simulator/src/CPUQubitsStatesGetter.cpp:34-157, glue.cpp:447-506
-//   prepareProbArray  CPUQubitsStatesGetter.cpp:159-254, DeviceProbArrayCalculator.cu
+//   prepareProbArray  (k_marginal.hip)
 //   SamplingPool      CPUSamplingPool.cpp:8-81: inclusive prefix sum, normalised by 1/sum,
 //                     observation = upper_bound(cdf, u), then zero bits inserted at the
 //                     empty-lane positions.
@@ -69,23 +69,6 @@ __global__ __launch_bounds__(kBlock) void k_get_states(void *out, const ViewArgs
         }
         static_cast<R *>(out)[j] = v;
     }
-}
-
-// out[x] = sum_h prod_q |a_q(x | h << n_lanes)|^2, FP64 accumulation.
-template <class R, class O>
-__global__ __launch_bounds__(kBlock) void k_prob_array(O *out, const ViewArgs g, int n_lanes,
-                                                       u64 n_hidden_states, u64 n_out, u64 first) {
-    const u64 x = first + (u64)blockIdx.x * kBlock + threadIdx.x;
-    if (x >= n_out) return;
-    double s = 0.;
-    for (u64 h = 0; h < n_hidden_states; ++h) {
-        const u64 ext = x | (h << n_lanes);
-        double v = 1.;
-        for (int q = 0; q < g.n_qs; ++q)
-            v *= abs2d(static_cast<const Cx<R> *>(g.ptr[q])[local_index(ext, g, q)]);
-        s += v;
-    }
-    out[x] = (O)s;
 }
 
 ViewArgs make_view(const std::vector<StateView> &qs) {
@@ -161,28 +144,8 @@ void get_states(void *host_out, int mathop, const std::vector<StateView> &qs, u6
     dev_free(d_out, 2 * dbytes);
 }
 
-template <class R>
-void launch_prob_array(void *d_out, bool out_double, const std::vector<StateView> &qs, int n_lanes,
-                       int n_hidden) {
-    const ViewArgs g = make_view(qs);
-    const u64 n_out = u64(1) << n_lanes;
-    const u64 n_h = u64(1) << n_hidden;
-    hipStream_t s = stream();
-    for_each_piece(n_out, [&](u64 first, u64 count) {
-        if (out_double)
-            hipLaunchKernelGGL((k_prob_array<R, double>), blocks_for(count), kBlock, 0, s,
-                               static_cast<double *>(d_out), g, n_lanes, n_h, n_out, first);
-        else
-            hipLaunchKernelGGL((k_prob_array<R, R>), blocks_for(count), kBlock, 0, s,
-                               static_cast<R *>(d_out), g, n_lanes, n_h, n_out, first);
-    });
-    QG_HIP(hipGetLastError());
-}
-
 template void get_states<float>(void *, int, const std::vector<StateView> &, u64, i64, i64, i64);
 template void get_states<double>(void *, int, const std::vector<StateView> &, u64, i64, i64, i64);
-template void launch_prob_array<float>(void *, bool, const std::vector<StateView> &, int, int);
-template void launch_prob_array<double>(void *, bool, const std::vector<StateView> &, int, int);
 
 // ---- sampling pool -----------------------------------------------------------------------
 //
@@ -191,9 +154,15 @@ template void launch_prob_array<double>(void *, bool, const std::vector<StateVie
 // Es / Ms / Hs the inclusive prefix of the previous chunk in the tile / tile in the
 // super-tile / super-tile in the pool (0 for the first), each computed serially:
 //     cdf(bin) = Hs + (Ms + (Es + P))          (each + rounded in FP64)
-// The value at the end of a chunk, tile or super-tile equals the stored inclusive prefix,
-// so cdf never decreases across a boundary. A draw u returns the first bin with
-// cdf * (1 / total) > u (upper_bound semantics), found level by level.
+// P at the end of a chunk is the chunk's serial sum, and the stored chunk prefixes are
+// scanned from those sums; the tile total stored for the upper levels is the tile's last
+// chunk prefix, and each upper level stores its own serial prefix, so the value at the end
+// of a tile or super-tile equals the stored inclusive prefix exactly. Inside a chunk the
+// association of the sums differs from the stored prefix by rounding, so a draw in that
+// ulp-wide gap past the chunk's last bin returns the chunk's last bin with p > 0. A draw u
+// returns the first bin with cdf * (1 / total) > u (upper_bound semantics), found level by
+// level. A bounded pool stores no p: a draw recomputes |a|^2 of the chunk's 16 bins from the
+// state in FP64, the same values the build summed, so its CDF is the same.
 
 namespace {
 
@@ -212,7 +181,7 @@ struct ProbSrc {};
 
 // One block per tile: p = |a|^2 (from the state) or p given; chunk sums; serial scan of the
 // 256 chunk sums; tile total; last bin with p > 0.
-template <class R, bool FROM_STATE>
+template <class R, bool FROM_STATE, bool STORE_P>
 __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
                                                       double *__restrict__ p, double *__restrict__ e_inc,
                                                       double *__restrict__ t_tot, i64 *__restrict__ t_last) {
@@ -243,7 +212,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restr
         const int li = r * kPoolThreads + threadIdx.x;
         const i64 bin = base + li;
         // probabilities from the state, and zeros for the padding up to whole tiles
-        if (FROM_STATE || bin >= n_bins) p[bin] = v[r];
+        if (STORE_P && (FROM_STATE || bin >= n_bins)) p[bin] = v[r];
         sp[li + li / kChunk] = v[r];
     }
     __syncthreads();
@@ -284,7 +253,8 @@ __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restr
         sc[4 * l + 1] = before + q1;
         sc[4 * l + 2] = before + q2;
         sc[4 * l + 3] = before + q3;
-        if (l == 63) t_tot[blockIdx.x] = incl;
+        // the tile total is its last chunk prefix, so the levels above agree with it exactly
+        if (l == 63) t_tot[blockIdx.x] = before + q3;
         if (l == 0) {
             i64 m = sl[0];
 #pragma unroll
@@ -340,11 +310,21 @@ __global__ void k_pool_top(double *h_inc, i64 n_supers, double *total, const i64
 
 struct SampleArgs {
     const double *p, *e_inc, *m_inc, *h_inc;
-    i64 n_chunks, n_tiles, n_supers;
+    const void *amps;                  // bounded pool: bins read as |a|^2 of the state
+    int amp_kind;                      // 0: p, 1: complex64 amplitudes, 2: complex128
+    const i64 *t_last, *s_last;        // last nonzero bin per tile / super-tile
+    i64 n, n_chunks, n_tiles, n_supers;
     double norm;
     i64 last_nonzero;
     BitInsert empty;
 };
+
+__device__ __forceinline__ double bin_prob(const SampleArgs &g, i64 bin) {
+    if (g.amp_kind == 0) return g.p[bin];
+    if (bin >= g.n) return 0.;   // padding up to whole tiles
+    if (g.amp_kind == 1) return abs2d(static_cast<const Cx<float> *>(g.amps)[bin]);
+    return abs2d(static_cast<const Cx<double> *>(g.amps)[bin]);
+}
 
 // first index in [lo, hi) with f(index) true, for f monotone false..true; hi if none
 template <class F>
@@ -373,41 +353,66 @@ __global__ __launch_bounds__(kBlock) void k_pool_sample(const SampleArgs g, cons
         const double hs = gsup > 0 ? g.h_inc[gsup - 1] : 0.;
         const i64 t0 = gsup * kSuperTiles;
         const i64 t1 = (t0 + kSuperTiles < g.n_tiles) ? t0 + kSuperTiles : g.n_tiles;
-        i64 t = first_true(t0, t1, [&](i64 k) { return (hs + g.m_inc[k]) * norm > x; });
-        if (t == t1) t = t1 - 1;
-        const double ms = t > t0 ? g.m_inc[t - 1] : 0.;
-        const i64 c0 = t * kTileChunks, c1 = c0 + kTileChunks;
-        i64 c = first_true(c0, c1, [&](i64 k) { return (hs + (ms + g.e_inc[k])) * norm > x; });
-        if (c == c1) c = c1 - 1;
-        const double es = c > c0 ? g.e_inc[c - 1] : 0.;
-        const double *pc = g.p + c * kChunk;
-        double run = 0.;
-        bin = c * kChunk + kChunk - 1;
-        for (int k = 0; k < kChunk; ++k) {
-            run = (k == 0) ? pc[0] : run + pc[k];
-            if ((hs + (ms + (es + run))) * norm > x) {
-                bin = c * kChunk + k;
-                break;
+        const i64 t = first_true(t0, t1, [&](i64 k) { return (hs + g.m_inc[k]) * norm > x; });
+        if (t == t1) {
+            // not reached: the super-tile's prefix is its last tile's (exactly equal sums)
+            bin = g.s_last[gsup];
+        } else {
+            const double ms = t > t0 ? g.m_inc[t - 1] : 0.;
+            const i64 c0 = t * kTileChunks, c1 = c0 + kTileChunks;
+            const i64 c = first_true(c0, c1, [&](i64 k) { return (hs + (ms + g.e_inc[k])) * norm > x; });
+            if (c == c1) {
+                // not reached: the tile's prefix is its last chunk's (exactly equal sums)
+                bin = g.t_last[t];
+            } else {
+                const double es = c > c0 ? g.e_inc[c - 1] : 0.;
+                double run = 0.;
+                i64 last = -1;
+                bin = -1;
+                for (int k = 0; k < kChunk; ++k) {
+                    const double pk = bin_prob(g, c * kChunk + k);
+                    run = (k == 0) ? pk : run + pk;
+                    if (pk > 0.) last = c * kChunk + k;
+                    if ((hs + (ms + (es + run))) * norm > x) {
+                        bin = c * kChunk + k;
+                        break;
+                    }
+                }
+                // u in the rounding gap past the chunk's serial sum: its last bin with p > 0
+                if (bin < 0) bin = last;
             }
         }
+        if (bin < 0) bin = g.last_nonzero;
     }
     obs[i] = (i64)insert_zero_bits((u64)bin, g.empty);
 }
 
 }  // namespace
 
-void pool_alloc(PoolData &pd, int n_lanes) {
+size_t pool_bytes(int n_lanes, bool with_p) {
+    const i64 n = i64(1) << n_lanes;
+    const i64 n_tiles = (n + kTile - 1) / kTile;
+    const i64 n_chunks = n_tiles * kTileChunks;
+    const i64 n_supers = (n_tiles + kSuperTiles - 1) / kSuperTiles;
+    const size_t n_p = with_p ? (size_t)n_tiles * kTile : 0;
+    const size_t n_m = (size_t)n_supers * kSuperTiles;
+    return sizeof(double) * (n_p + n_chunks + n_m + n_supers + 8) + sizeof(i64) * (n_tiles + n_supers);
+}
+
+void pool_alloc(PoolData &pd, int n_lanes, bool with_p) {
     pd.n = i64(1) << n_lanes;
     pd.n_tiles = (pd.n + kTile - 1) / kTile;
     pd.n_chunks = pd.n_tiles * kTileChunks;
     pd.n_supers = (pd.n_tiles + kSuperTiles - 1) / kSuperTiles;
-    const size_t n_p = (size_t)pd.n_tiles * kTile;
+    const size_t n_p = with_p ? (size_t)pd.n_tiles * kTile : 0;
     const size_t n_m = (size_t)pd.n_supers * kSuperTiles;
-    pd.bytes = sizeof(double) * (n_p + pd.n_chunks + n_m + pd.n_supers + 8) + sizeof(i64) * (pd.n_tiles + pd.n_supers);
+    pd.bytes = pool_bytes(n_lanes, with_p);
     pd.block = dev_alloc(pd.bytes);
     double *d = static_cast<double *>(pd.block);
-    pd.p = d;
-    pd.e_inc = pd.p + n_p;
+    pd.p = with_p ? d : nullptr;
+    pd.amps = nullptr;
+    pd.amp_kind = 0;
+    pd.e_inc = d + n_p;
     pd.m_inc = pd.e_inc + pd.n_chunks;
     pd.h_inc = pd.m_inc + n_m;
     pd.d_last_nonzero = reinterpret_cast<i64 *>(pd.h_inc + pd.n_supers);
@@ -422,11 +427,11 @@ void pool_free(PoolData &pd) {
 
 namespace {
 
-template <class R, bool FROM_STATE>
+template <class R, bool FROM_STATE, bool STORE_P>
 void pool_build(PoolData &pd, const Cx<R> *a) {
     hipStream_t s = stream();
     double *d_total = pd.h_inc + pd.n_supers + 1;
-    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE>), (unsigned)pd.n_tiles, kPoolThreads, 0, s, a, pd.n,
+    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE, STORE_P>), (unsigned)pd.n_tiles, kPoolThreads, 0, s, a, pd.n,
                        pd.p, pd.e_inc, pd.m_inc, pd.t_last);
     QG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc, pd.t_last,
@@ -449,22 +454,29 @@ void pool_build(PoolData &pd, const Cx<R> *a) {
 }  // namespace
 
 template <class R>
-void pool_build_from_state(PoolData &pd, const Cx<R> *a, int n_lanes) {
-    pool_alloc(pd, n_lanes);
+void pool_build_from_state(PoolData &pd, const Cx<R> *a, int n_lanes, bool bounded) {
+    pool_alloc(pd, n_lanes, !bounded);
     try {
-        pool_build<R, true>(pd, a);
+        if (bounded) {
+            pool_build<R, true, false>(pd, a);
+            pd.amps = a;
+            pd.amp_kind = sizeof(R) == 4 ? 1 : 2;
+        } else {
+            pool_build<R, true, true>(pd, a);
+        }
     } catch (...) {
         pool_free(pd);
         throw;
     }
 }
 
-template void pool_build_from_state<float>(PoolData &, const Cx<float> *, int);
-template void pool_build_from_state<double>(PoolData &, const Cx<double> *, int);
+template void pool_build_from_state<float>(PoolData &, const Cx<float> *, int, bool);
+template void pool_build_from_state<double>(PoolData &, const Cx<double> *, int, bool);
 
 void pool_build_from_probs(PoolData &pd, int n_lanes) {
     (void)n_lanes;
-    pool_build<double, false>(pd, nullptr);
+    QG_REQUIRE(pd.p != nullptr, "pool without a probability array");
+    pool_build<double, false, true>(pd, nullptr);
 }
 
 void pool_sample(const PoolData &pd, i64 *host_obs, i64 n_samples, const double *host_u,
@@ -475,6 +487,11 @@ void pool_sample(const PoolData &pd, i64 *host_obs, i64 n_samples, const double 
     g.e_inc = pd.e_inc;
     g.m_inc = pd.m_inc;
     g.h_inc = pd.h_inc;
+    g.amps = pd.amps;
+    g.amp_kind = pd.amp_kind;
+    g.t_last = pd.t_last;
+    g.s_last = pd.s_last;
+    g.n = pd.n;
     g.n_chunks = pd.n_chunks;
     g.n_tiles = pd.n_tiles;
     g.n_supers = pd.n_supers;

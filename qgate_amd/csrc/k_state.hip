@@ -9,8 +9,10 @@
 //   applyReset            :286-305  a[i0] = a[i1], a[i1] = 0
 // Every pass streams 16-byte units (2 complex64 or 1 complex128 amplitudes) with
 // non-temporal hints. Probability sums accumulate in FP64 whatever the state precision, in a
-// fixed order (per-block partials, then one block over the partials), so repeated calls on
-// the same state return the same bits.
+// fixed order (per-block partials, then one block over the partials), so repeated calc_prob
+// calls on the same state return the same bits. decohere_prob (a recorded collapse fused with
+// the next probability) sums per-wavefront partials instead: the same terms in another fixed
+// order, so its result may differ from a separate calc_prob in the last bits.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -701,6 +703,43 @@ void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, doubl
         hipLaunchKernelGGL((k_pauli_decohere<R>), grid_for(count, kWave), kWave, 0, s, a, P, value, rnorm, half,
                            n_items, first);
     });
+    QG_HIP(hipGetLastError());
+}
+
+namespace {
+
+// dst = src over n_units 16-byte units, kCopyUnits units per thread with every load in flight
+// before the stores, non-temporal both ways: the streaming ceiling a state pass can reach.
+constexpr int kCopyUnits = 4;
+
+__global__ __launch_bounds__(kBlock) void k_stream_copy(f32x4 *__restrict__ dst, const f32x4 *__restrict__ src,
+                                                        u64 n_units, u64 first) {
+    const u64 base = first + (u64)blockIdx.x * (kBlock * kCopyUnits) + threadIdx.x;
+    f32x4 v[kCopyUnits];
+#pragma unroll
+    for (int r = 0; r < kCopyUnits; ++r) {
+        const u64 u = base + (u64)r * kBlock;
+        if (u < n_units) v[r] = __builtin_nontemporal_load(src + u);
+    }
+#pragma unroll
+    for (int r = 0; r < kCopyUnits; ++r) {
+        const u64 u = base + (u64)r * kBlock;
+        if (u < n_units) __builtin_nontemporal_store(v[r], dst + u);
+    }
+}
+
+}  // namespace
+
+void launch_stream_copy(void *dst, const void *src, u64 bytes) {
+    QG_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
+    const u64 n_units = bytes / 16;
+    const u64 per_block = (u64)kBlock * kCopyUnits;
+    hipStream_t s = stream();
+    // pieces of whole blocks keep every grid below 2^32 threads
+    const u64 piece = launch_piece_items();
+    for (u64 first = 0; first < n_units; first += piece)
+        hipLaunchKernelGGL(k_stream_copy, (unsigned)((std::min(piece, n_units - first) + per_block - 1) / per_block),
+                           kBlock, 0, s, static_cast<f32x4 *>(dst), static_cast<const f32x4 *>(src), n_units, first);
     QG_HIP(hipGetLastError());
 }
 

@@ -78,6 +78,8 @@ double pauli_expectation(const Cx<R> *psi, Cx<R> *lam, int n_lanes, const std::v
 
 // ---- state -----------------------------------------------------------------------------
 template <class R> void launch_set_zero_state(Cx<R> *a, int n_lanes);
+// dst = src (bytes a multiple of 16): the non-temporal streaming copy used as the HBM ceiling.
+void launch_stream_copy(void *dst, const void *src, u64 bytes);
 // P(bit `lane` == 0), FP64 accumulation; synchronises.
 template <class R> double calc_prob(const Cx<R> *a, int n_lanes, int lane);
 // Doubles of partial-sum buffer calc_prob_in needs for this lane (the partials, run sums and
@@ -124,7 +126,10 @@ void launch_prob_array(void *d_out, bool out_double, const std::vector<StateView
 // ---- sampling pool ---------------------------------------------------------------------
 struct PoolData {
     i64 n = 0;                 // number of bins (2^n_lanes)
-    double *p = nullptr;       // bin probabilities (unnormalised), padded to whole chunks
+    double *p = nullptr;       // bin probabilities (unnormalised), padded to whole chunks;
+                               // null for a bounded pool, which reads them from `amps`
+    const void *amps = nullptr;   // bounded pool: the state vector, |a|^2 taken in FP64
+    int amp_kind = 0;             // 0: p array, 1: complex64 amplitudes, 2: complex128
     double *e_inc = nullptr;   // per 16-bin chunk: inclusive prefix within its tile
     double *m_inc = nullptr;   // per 4096-bin tile: inclusive prefix within its super-tile
     double *h_inc = nullptr;   // per 2^22-bin super-tile: inclusive prefix over the pool
@@ -137,10 +142,15 @@ struct PoolData {
     size_t bytes = 0;
     void *block = nullptr;     // single device allocation holding the arrays
 };
+// bounded: keep no probability array (2^n_lanes / 16 doubles of prefix sums instead of
+// 2^n_lanes + that); draws then read |a|^2 of the found chunk from the state vector, which
+// must not change while the pool is used.
 template <class R>
-void pool_build_from_state(PoolData &pd, const Cx<R> *a, int n_lanes);
+void pool_build_from_state(PoolData &pd, const Cx<R> *a, int n_lanes, bool bounded);
 void pool_build_from_probs(PoolData &pd, int n_lanes);   // pd.p already filled
-void pool_alloc(PoolData &pd, int n_lanes);
+// Device bytes of a pool over n_lanes lanes, with or without its probability array.
+size_t pool_bytes(int n_lanes, bool with_p);
+void pool_alloc(PoolData &pd, int n_lanes, bool with_p = true);
 void pool_free(PoolData &pd);
 void pool_sample(const PoolData &pd, i64 *host_obs, i64 n_samples, const double *host_u,
                  const std::vector<int> &empty_lanes);

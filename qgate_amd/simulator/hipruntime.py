@@ -61,6 +61,7 @@ def lib():
         'qghip_device_info': (ctypes.c_int, [_c_int_p, _c_int_p, ctypes.POINTER(ctypes.c_uint64)]),
         'qghip_set_launch_piece': (ctypes.c_int, [ctypes.c_int64]),
         'qghip_set_register_tiles': (ctypes.c_int, [ctypes.c_int]),
+        'qghip_set_sampling_pool_workspace': (ctypes.c_int, [ctypes.c_int]),
         'qghip_register_matrix_factory': (ctypes.c_int, [ctypes.c_char_p, _c_int_p]),
         'qghip_gate_matrix': (ctypes.c_int, [ctypes.c_int, _c_double_p, ctypes.c_int, ctypes.c_int, _c_double_p]),
         'qghip_qubit_states_new': (ctypes.c_int, [ctypes.c_int, _c_void_pp]),
@@ -96,6 +97,7 @@ def lib():
         'qghip_sampling_pool_sample': (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p]),
         'qghip_sampling_pool_delete': (ctypes.c_int, [H]),
         'qghip_profile_gate': (ctypes.c_int, [H, H, _c_double_p, _c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p]),
+        'qghip_profile_copy': (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, _c_double_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -200,6 +202,28 @@ def device_info():
 def set_register_tiles(on):
     """Full-size tile passes in registers (True, default) or in LDS (False)."""
     check(lib().qghip_set_register_tiles(1 if on else 0))
+
+
+_POOL_WORKSPACE = {'auto': 0, 'bounded': 1, 'full': 2}
+
+
+def set_sampling_pool_workspace(mode='auto'):
+    """Workspace of sampling pools created from now on: 'full' keeps the FP64 probability of
+    every bin; 'bounded' keeps only prefix sums per 16 bins (a pool over a whole state vector
+    in its own qubit order then reads the amplitudes when it draws, and fails if that state
+    vector has changed since); 'auto' is 'full' unless the probability array would not fit
+    in free device memory. Draws are identical in every mode."""
+    if mode not in _POOL_WORKSPACE:
+        raise ValueError('pool workspace must be one of {}'.format(sorted(_POOL_WORKSPACE)))
+    check(lib().qghip_set_sampling_pool_workspace(_POOL_WORKSPACE[mode]))
+
+
+def profile_copy(nbytes, reps=5):
+    """Mean duration (ms) of the library's streaming copy kernel over nbytes (read + write =
+    2 * nbytes per run), timed with HIP events."""
+    out = ctypes.c_double()
+    check(lib().qghip_profile_copy(int(nbytes), int(reps), ctypes.byref(out)))
+    return out.value
 
 
 def set_launch_piece(items):
