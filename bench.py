@@ -14,12 +14,22 @@ uncontrolled 2x2 gate 2 S 2^n, c-controlled 2 S 2^(n-c), diagonal gates with m00
 roofline: the streaming 2x2 pair kernel on uncontrolled U3 gates at 30 qubits, timed with
 HIP events on the library stream over every target bit; achieved = 2 S 2^30 bytes / mean
 launch time, against 8 TB/s; traffic = HBM bytes per launch measured with the FETCH_SIZE and
-WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md); copy_GBs = the HIP runtime's own
-device-to-device copy of 4 GiB (read + write bytes, best of 5), and frac_of_copy =
-achieved / copy_GBs.
+WRITE_SIZE counters (profiles/traffic_gate_30q_c64.md); copy_GBs = the library's own
+non-temporal streaming copy kernel over 4 GiB (read + write bytes, mean of 5 launches timed
+with HIP events), and frac_of_copy = achieved / copy_GBs.
+
+complex64_vs_complex128: the workload applied twice to |0...0> in complex64 and in complex128;
+max |difference| over a strided slice of 2^20 amplitudes and over P(qubit = 0) of every qubit
+(the error figure of the complex64 path at 30 qubits).
+
+marginals_30q: probability arrays with hidden qubits on the 30-qubit state (the sum over
+hidden qubits that pools over some qubits build): 1 of 30 kept, and 26 of 30 kept with the 4
+lowest hidden; read bytes 8 * 2^30 per build.
 
 large_state_34q: a 34-qubit complex64 state (128 GiB): per-target gate GB/s split at target 7,
-P(0) throughput and a 30-of-34-qubit sampling-pool build (prefix sums over 2^34 amplitudes).
+P(0) throughput, a 30-of-34 and a 1-of-34 sampling-pool build (marginal over the hidden
+qubits, then prefix sums), and a bounded pool over all 34 qubits (prefix sums only) with
+2^20 draws.
 
 batched_adjoint: 4096 ansatz circuits of 8..20 qubits (10 layers, complex64), each with its
 own parameters and observable, <H> and gradients in one graph replay, against a loop of
@@ -42,16 +52,20 @@ complex128_step: the same unfused step on a complex128 state (16 GiB), one timed
 
 fused_dense: the same step with runs of gates fused into dense unitaries of at most k = 2..6
 qubits (built in FP64, applied as one kernel each; complex64 blocks of 5-6 qubits on FP32
-matrix cores), and single 5- and 6-qubit blocks on the 30-qubit state with their GB/s,
-TFLOP/s and fraction of the 157.3 TFLOP/s FP32 matrix-core peak.
+matrix cores, smaller blocks and complex128 blocks of up to 5 qubits staged through LDS), and
+single blocks of k = 2..6 qubits on the 30-qubit state in complex64 and complex128 with GB/s,
+TFLOP/s and the fraction of max(bytes / 8 TB/s, flops / dense peak of the precision).
 
 cpu_baseline: the reference's own C++ CPU runtime (oracle/_ref, or the C restatement when
 that is not built) applying the first gates of the same circuit to a 30-qubit complex64
-state in host memory, for about 15 s of work, all host cores.
+state in host memory, for about 15 s of work, all host cores; then one worker
+(QGATE_NUM_WORKERS=1, in a child process: the count is read once per process) for about 10 s,
+the CPU model and the host memory copy rate (numpy copy of 2 GiB, read + write, best of 3).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batch] [--no-large]
     python bench.py --batch-only      # the batched adjoint alone, one JSON object
     python bench.py --dense-only      # dense 5- and 6-qubit blocks alone, one JSON object
+    python bench.py --marginal-only   # marginals and pools on the 30- and 34-qubit states
 """
 import argparse
 import json
@@ -114,6 +128,7 @@ class Runner:
 
 
 def cpu_baseline(gates, n, amp, budget_s=15.0):
+    """Gate GB/s of the reference CPU core on the first gates of the workload (see module doc)."""
     from oracle import cpuruntime
     if cpuruntime.have_reference():
         module, kind = cpuruntime.reference(), 'reference'
@@ -132,8 +147,42 @@ def cpu_baseline(gates, n, amp, budget_s=15.0):
     dt = time.perf_counter() - t0
     r.delete()
     return {'value': round(nbytes / dt / 1e9, 3), 'unit': 'GB/s', 'cores': cores, 'kind': kind,
-            'sample': 'first {} of the 2000 gates (30 qubits, complex64, host memory), {:.1f} s, '
-                      '{:.3f} s/gate'.format(done, dt, dt / max(done, 1))}
+            'sample': 'first {} of the {} gates ({} qubits, complex64, host memory), {:.1f} s, '
+                      '{:.3f} s/gate'.format(done, len(gates), n, dt, dt / max(done, 1))}
+
+
+def host_facts():
+    """CPU model and the host memory copy rate (numpy copy of 2 GiB, read + write, best of 3)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    a = np.ones(1 << 28)
+    b = np.empty_like(a)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        np.copyto(b, a)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {'cpu_model': model, 'host_copy_GBs': round(2 * a.nbytes / best / 1e9, 1),
+            'host_threads_visible': len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline_one_worker(budget_s=10.0):
+    """The CPU baseline with QGATE_NUM_WORKERS=1, run in a child process."""
+    import subprocess
+    env = dict(os.environ, QGATE_NUM_WORKERS='1')
+    cmd = [sys.executable, os.path.abspath(__file__), '--cpu-only', '--cpu-budget', str(budget_s)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {'error': r.stderr[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def measured_traffic():
@@ -184,9 +233,27 @@ def roofline(runner, n, amp, reps=4):
 MFMA_FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix-core peak
 
 
+def _strided_slice(module, r, n, dtype, count=1 << 20):
+    """count amplitudes of runner r's state, spread over all 2^n (stride 2^n // count + 1)."""
+    from qgate_amd.simulator import qubits as qmod
+    from qgate_amd.simulator.lanes import Lane
+    getter = module.create_qubits_states_getter(dtype)
+    lanes = []
+    for i in range(n):
+        ln = Lane(None, i)
+        ln.set_external(i)
+        lanes.append(ln)
+    step = (1 << n) // count + 1
+    cnt = ((1 << n) - 1) // step + 1
+    out = np.empty(cnt, np.complex64 if dtype == np.float32 else np.complex128)
+    getter.get_states(out, 0, qmod.null, [(r.qs, lanes)], [], cnt, 0, step)
+    return out.astype(np.complex128)
+
+
 def c128_step_timing(module, n, gates):
     """The same unfused step on a complex128 state (16 GiB at 30 qubits): one warm-up, one
-    timed run; algorithmic bytes with S = 16."""
+    timed run; algorithmic bytes with S = 16. Then the complex64 state of the same two runs
+    from |0...0> against it (complex64_vs_complex128)."""
     r = Runner(module, n, np.float64)
     r.apply(gates)
     r.sync()
@@ -194,17 +261,29 @@ def c128_step_timing(module, n, gates):
     r.apply(gates)
     r.sync()
     dt = time.perf_counter() - t0
+    ref = _strided_slice(module, r, n, np.float64)
+    ref_p0 = [r.proc.calc_probability(r.qs, q) for q in range(n)]
     r.delete()
+    r32 = Runner(module, n, np.float32)
+    r32.apply(gates)
+    r32.apply(gates)
+    r32.sync()
+    got = _strided_slice(module, r32, n, np.float32)
+    p0 = [r32.proc.calc_probability(r32.qs, q) for q in range(n)]
+    r32.delete()
     step_bytes = sum(gate_bytes(k, n, 16) for k, _, _ in gates)
+    err = {'max_abs_amplitude_diff': float(np.max(np.abs(got - ref))),
+           'amplitudes_compared': len(ref),
+           'max_abs_p0_diff': float(np.max(np.abs(np.array(p0) - np.array(ref_p0)))),
+           'max_abs_amplitude': float(np.max(np.abs(ref)))}
     return {'ms_per_step': round(dt * 1e3, 1), 'GBs': round(step_bytes / dt / 1e9, 1),
             'frac_of_hbm_peak': round(step_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
-            'gates_per_s': round(len(gates) / dt, 1)}
+            'gates_per_s': round(len(gates) / dt, 1)}, err
 
 
 def dense_timing(r, gates, step_s, step_bytes, n, widths=(2, 3, 4, 5, 6)):
     """The step with dense-block fusion at the given widths (one warm-up, one timed run each),
-    and single dense blocks of 5 and 6 qubits on the 30-qubit complex64 state (the matrix-core
-    kernel): 2 S 2^n bytes and 8 2^k 2^n flops per block, best of 3 synchronised calls."""
+    then single dense blocks (dense_block_timing)."""
     from qgate_amd.simulator import hipruntime
     steps = {}
     for k in widths:
@@ -220,67 +299,55 @@ def dense_timing(r, gates, step_s, step_bytes, n, widths=(2, 3, 4, 5, 6)):
                     'speedup_vs_unfused': round(step_s / dt, 2)}
     hipruntime.set_fusion('off')
     r.proc.set_fusion('off')
-    rs = np.random.RandomState(17)
-    blocks = {}
-    for k, lanes in ((5, [2, 9, 15, 21, 27]), (6, [1, 6, 12, 17, 23, 28])):
-        z = rs.normal(size=(1 << k, 1 << k)) + 1j * rs.normal(size=(1 << k, 1 << k))
-        u, _ = np.linalg.qr(z)
-        best = None
-        for _ in range(3):
-            t0 = time.perf_counter()
-            r.proc.apply_unitary(r.qs, lanes, u)
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        tflops = 8 * (1 << k) * (1 << n) / best / 1e12
-        blocks[k] = {'lanes': lanes, 'ms': round(best * 1e3, 3), 'GBs': round(2 * 8 * (1 << n) / best / 1e9, 1),
-                     'TFLOPs': round(tflops, 1), 'frac_of_mfma_fp32_peak': round(tflops / MFMA_FP32_PEAK_TFLOPS, 3)}
-    return {'steps': steps, 'blocks': blocks}
+    return {'steps': steps, 'blocks': dense_block_timing(n, r)}
+
+
+DENSE_LANES = {2: [3, 17], 3: [2, 14, 25], 4: [0, 9, 18, 27], 5: [2, 9, 15, 21, 27], 6: [1, 6, 12, 17, 23, 28]}
+PEAK_TFLOPS = {np.float32: 157.3, np.float64: 78.6}   # MI355X dense FP32 / FP64
+
+
+def dense_block_timing(n, r32=None, widths=(2, 3, 4, 5, 6)):
+    """Single dense blocks of k = 2..6 qubits on the n-qubit state in complex64 and complex128
+    (best of 3 synchronised calls): 2 S 2^n bytes and 8 2^k 2^n flops per block, against
+    bound = max(bytes / 8 TB/s, flops / peak) with the MI355X dense peak of the precision
+    (FP32 157.3, FP64 78.6 TFLOP/s)."""
+    from qgate_amd.simulator import hipruntime
+    out = {}
+    for dtype, amp in ((np.float32, 8), (np.float64, 16)):
+        r = r32 if (dtype == np.float32 and r32 is not None) else Runner(hipruntime, n, dtype)
+        rs = np.random.RandomState(17)
+        res = {}
+        for k in widths:
+            lanes = DENSE_LANES[k]
+            z = rs.normal(size=(1 << k, 1 << k)) + 1j * rs.normal(size=(1 << k, 1 << k))
+            u, _ = np.linalg.qr(z)
+            best = None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                r.proc.apply_unitary(r.qs, lanes, u)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            nbytes, flops = 2 * amp * (1 << n), 8 * (1 << k) * (1 << n)
+            bound = max(nbytes / (HBM_PEAK_GBS * 1e9), flops / (PEAK_TFLOPS[dtype] * 1e12))
+            res[k] = {'lanes': lanes, 'ms': round(best * 1e3, 3), 'GBs': round(nbytes / best / 1e9, 1),
+                      'TFLOPs': round(flops / best / 1e12, 1), 'bound_ms': round(bound * 1e3, 3),
+                      'bound': 'hbm' if nbytes / (HBM_PEAK_GBS * 1e9) >= flops / (PEAK_TFLOPS[dtype] * 1e12)
+                      else 'flops', 'frac_of_bound': round(bound / best, 3)}
+        if r is not r32:
+            r.delete()
+        out['complex64' if dtype == np.float32 else 'complex128'] = res
+    return out
+
+
 
 
 def copy_rate_GBs(gib=4, reps=5):
-    """Measured device-to-device copy rate (the HIP runtime's own copy of a gib-GiB buffer,
-    hipMemcpyAsync on the null stream timed with HIP events, best of reps; bytes = read +
-    write): what a plain copy reaches on this HBM."""
-    import ctypes
-    hip = ctypes.CDLL('libamdhip64.so')
-    vp, sz = ctypes.c_void_p, ctypes.c_size_t
-    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
-    hip.hipFree.argtypes = [vp]
-    hip.hipMemsetAsync.argtypes = [vp, ctypes.c_int, sz, vp]
-    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
-    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
-    hip.hipEventRecord.argtypes = [vp, vp]
-    hip.hipEventSynchronize.argtypes = [vp]
-    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
-    hip.hipEventDestroy.argtypes = [vp]
-
-    def ok(rc):
-        if rc != 0:
-            raise RuntimeError('HIP error {}'.format(rc))
-
+    """The library's non-temporal streaming copy kernel over gib GiB (mean of reps launches,
+    HIP events on the library stream; bytes = read + write): the streaming ceiling of HBM."""
+    from qgate_amd.simulator import hipruntime
     nbytes = gib << 30
-    src, dst, e0, e1 = vp(), vp(), vp(), vp()
-    ok(hip.hipMalloc(ctypes.byref(src), nbytes))
-    ok(hip.hipMalloc(ctypes.byref(dst), nbytes))
-    ok(hip.hipEventCreate(ctypes.byref(e0)))
-    ok(hip.hipEventCreate(ctypes.byref(e1)))
-    try:
-        ok(hip.hipMemsetAsync(src, 1, nbytes, None))
-        best = None
-        for _ in range(reps + 1):   # the first copy warms up
-            ok(hip.hipEventRecord(e0, None))
-            ok(hip.hipMemcpyAsync(dst, src, nbytes, 3, None))   # hipMemcpyDeviceToDevice
-            ok(hip.hipEventRecord(e1, None))
-            ok(hip.hipEventSynchronize(e1))
-            ms = ctypes.c_float()
-            ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
-            best = ms.value if best is None else min(best, ms.value)
-    finally:
-        hip.hipEventDestroy(e0)
-        hip.hipEventDestroy(e1)
-        hip.hipFree(src)
-        hip.hipFree(dst)
-    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
+    ms = hipruntime.profile_copy(nbytes, reps)
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
 
 
 def device_copy_ms(module, n):
@@ -449,6 +516,40 @@ def qasm_qft_timing(n=30):
             'run_ms': round((t2 - t1) * 1e3, 2), 'p0_qubit0': p0, 'p0_ms': round((t3 - t2) * 1e3, 2)}
 
 
+def pool_build_ms(getter, qs, n, kept, reps=3):
+    """Build time (best of reps, ms) of a device sampling pool over the lanes `kept` of an
+    n-lane state vector, in that order, the other lanes hidden."""
+    from qgate_amd.simulator.lanes import Lane
+    pos = {lane: i for i, lane in enumerate(kept)}
+    best = None
+    for _ in range(reps):
+        lanes = []
+        for i in range(n):
+            ln = Lane(None, i)
+            ln.set_external(pos.get(i, -1))
+            lanes.append(ln)
+        t0 = time.perf_counter()
+        pool = getter.create_sampling_pool(list(range(len(kept))), len(kept), n - len(kept), [(qs, lanes)], [])
+        dt = time.perf_counter() - t0
+        pool.delete()
+        best = dt if best is None else min(best, dt)
+    return best * 1e3
+
+
+def marginal_timing(r, n, amp):
+    """Pools over some qubits of runner r's n-qubit state (the hidden qubits summed by the
+    marginal kernels, then prefix sums): GB/s = amp * 2^n read bytes / build time."""
+    from qgate_amd.simulator import hipruntime
+    getter = hipruntime.create_qubits_states_getter(np.float32)
+    out = {}
+    for name, kept in (('1_of_{}'.format(n), [5]),
+                       ('{}_of_{}_low_hidden'.format(n - 4, n), list(range(4, n))),
+                       ('{}_of_{}_high_hidden'.format(n - 4, n), list(range(n - 4)))):
+        ms = pool_build_ms(getter, r.qs, n, kept)
+        out[name] = {'ms': round(ms, 2), 'read_GBs': round((amp << n) / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def large_state_timing(n=34):
     """Config of the large single-GPU state: 34 qubits complex64 (128 GiB). H on every qubit,
     then per-target timing of an uncontrolled gate (HIP events, 2 launches per target), P(0)
@@ -482,6 +583,26 @@ def large_state_timing(n=34):
     pool = getter.create_sampling_pool(list(range(30)), 30, n - 30, [(r.qs, lanes)], [])
     pool_s = time.perf_counter() - t0
     pool.delete()
+    pool_s = min(pool_s, pool_build_ms(getter, r.qs, n, list(range(30)), reps=2) * 1e-3)
+    one_ms = pool_build_ms(getter, r.qs, n, [5], reps=2)
+    # a bounded pool over all 34 qubits: prefix sums only (8 GiB), |a|^2 read at draw time
+    hipruntime.set_sampling_pool_workspace('bounded')
+    try:
+        all_lanes = []
+        for i in range(n):
+            ln = Lane(None, i)
+            ln.set_external(i)
+            all_lanes.append(ln)
+        t0 = time.perf_counter()
+        pool = getter.create_sampling_pool(list(range(n)), n, 0, [(r.qs, all_lanes)], [])
+        bounded_s = time.perf_counter() - t0
+        u = np.random.RandomState(1034).random_sample(1 << 20)
+        t0 = time.perf_counter()
+        pool.sample(len(u), u)
+        draws_s = time.perf_counter() - t0
+        pool.delete()
+    finally:
+        hipruntime.set_sampling_pool_workspace('auto')
     r.delete()
     return {'qubits': n, 'dtype': 'complex64', 'state_GiB': (amp << n) >> 30,
             'h_all_ms': round(h_s * 1e3, 1),
@@ -490,9 +611,16 @@ def large_state_timing(n=34):
             'per_target_GBs': [round(v) for v in gbs],
             'prob_ms_per_qubit': round(prob_s * 1e3, 2),
             'prob_GBs': round((amp << (n - 1)) / prob_s / 1e9, 1),
-            'min_prob0_after_h': round(min(probs), 6), 'max_prob0_after_h': round(max(probs), 6),
+            'prob0_state': 'H on every qubit, then the 68 timing launches of U(0.3, 1.1, 2.3)',
+            'min_prob0': round(min(probs), 6), 'max_prob0': round(max(probs), 6),
             'pool_build_30_of_34_ms': round(pool_s * 1e3, 1),
-            'pool_build_GBs': round((amp << n) / pool_s / 1e9, 1)}
+            'pool_build_GBs': round((amp << n) / pool_s / 1e9, 1),
+            'pool_build_1_of_34_ms': round(one_ms, 1),
+            'pool_build_1_of_34_GBs': round((amp << n) / (one_ms * 1e-3) / 1e9, 1),
+            'bounded_pool_34_of_34': {'build_ms': round(bounded_s * 1e3, 1),
+                                      'build_GBs': round((amp << n) / bounded_s / 1e9, 1),
+                                      'workspace_GiB': round(((1 << n) // 16 * 8) / 2 ** 30, 2),
+                                      'draws_2p20_ms': round(draws_s * 1e3, 2)}}
 
 
 def main():
@@ -511,6 +639,17 @@ def main():
     ap.add_argument('--dense-only', action='store_true',
                     help='only the dense 5- and 6-qubit blocks on the 30-qubit state; prints their JSON object '
                          '(for profiler runs)')
+    ap.add_argument('--marginal-only', action='store_true',
+                    help='only the marginal and pool timings on the 30-qubit and 34-qubit states')
+    ap.add_argument('--tile-only', action='store_true',
+                    help='only the workload step with tile-pass fusion (one warm-up, one timed step); prints '
+                         'its JSON object (for counter runs)')
+    ap.add_argument('--adjoint-only', action='store_true',
+                    help='only the 24-qubit adjoint forward and gradient (complex64 and complex128); prints '
+                         'their JSON object (for counter runs)')
+    ap.add_argument('--cpu-only', action='store_true',
+                    help='only the CPU baseline (no GPU use); prints its JSON object')
+    ap.add_argument('--cpu-budget', type=float, default=15.0)
     ap.add_argument('--roofline-only', action='store_true',
                     help='only the per-target gate timing loop and a 2 GiB copy kernel (for counter runs)')
     args = ap.parse_args()
@@ -519,8 +658,17 @@ def main():
               'across GPUs (got --gpus {}).'.format(args.gpus), file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_only:
+        from qgate_amd import workloads
+        gates = workloads.random_gate_list(args.qubits, args.gates, seed=2)
+        print(json.dumps(cpu_baseline(gates, args.qubits, 8, budget_s=args.cpu_budget)), flush=True)
+        return
     if args.batch_only:
         print(json.dumps(batch_timing(loop_sample=32)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({name: adjoint_timing(dt, reps=3) for name, dt in
+                          (('complex64', np.float32), ('complex128', np.float64))}), flush=True)
         return
 
     from qgate_amd import workloads
@@ -528,20 +676,39 @@ def main():
 
     n, amp = args.qubits, 8
     if args.dense_only:
-        r = Runner(hipruntime, n, np.float32)
-        print(json.dumps(dense_timing(r, [], 1., 0., n, widths=())), flush=True)
-        r.delete()
+        print(json.dumps(dense_block_timing(n)), flush=True)
         return
     gates = workloads.random_gate_list(n, args.gates, seed=2)
     step_bytes = sum(gate_bytes(k, n, amp) for k, _, _ in gates)
     info = hipruntime.device_info()
 
     r = Runner(hipruntime, n, np.float32)
+    if args.tile_only:
+        hipruntime.set_fusion('tile')
+        r.proc.set_fusion('tile')
+        r.apply(gates)
+        r.sync()
+        t0 = time.perf_counter()
+        r.apply(gates)
+        r.sync()
+        dt = time.perf_counter() - t0
+        hipruntime.set_fusion('off')
+        r.delete()
+        print(json.dumps({'fused_tile_ms': round(dt * 1e3, 2), 'gates': len(gates)}), flush=True)
+        return
+    if args.marginal_only:
+        r.apply(gates[:200])
+        r.sync()
+        out = {'marginals_30q': marginal_timing(r, n, amp)}
+        r.delete()
+        out['large_state_34q'] = large_state_timing()
+        print(json.dumps(out), flush=True)
+        return
     if args.roofline_only:
         roof = roofline(r, n, amp)
         r.delete()
         copy = device_copy_ms(hipruntime, 28)
-        print(json.dumps({'roofline': roof, 'copy': copy}))
+        print(json.dumps({'roofline': roof, 'copy': copy, 'stream_copy_GBs': copy_rate_GBs()}))
         return
     print('[bench] device {} ({} CUs), {} qubits complex64, {} gates, {:.1f} GB per step'
           .format(info['device_id'], info['n_cus'], n, len(gates), step_bytes / 1e9), file=sys.stderr, flush=True)
@@ -598,6 +765,8 @@ def main():
                          'GBs': round(out.nbytes / min(ts) / 1e9, 1)}
         del out
     print('[bench] readout: {}'.format(readout), file=sys.stderr, flush=True)
+    marginals = marginal_timing(r, n, amp)
+    print('[bench] marginals: {}'.format(marginals), file=sys.stderr, flush=True)
     u = np.random.RandomState(1002).random_sample(1 << 20)
     t2 = time.perf_counter()
     pool = getter.create_sampling_pool(list(range(n)), n, 0, [(r.qs, lanes)], [])
@@ -615,7 +784,7 @@ def main():
     roof = roofline(r, n, amp)
     r.delete()
     roof['copy_GBs'] = copy_rate_GBs()
-    c128 = c128_step_timing(hipruntime, n, gates)
+    c128, c64_err = c128_step_timing(hipruntime, n, gates)
     print('[bench] complex128 step: {}'.format(c128), file=sys.stderr, flush=True)
     roof['frac_of_copy'] = round(roof['achieved'] / roof['copy_GBs'], 4)
     print('[bench] roofline: {:.1f} GB/s, device copy {:.1f} GB/s'.format(roof['achieved'], roof['copy_GBs']),
@@ -651,6 +820,8 @@ def main():
                        'speedup_vs_unfused': round(step_s / fused_s, 2)},
         'fused_dense': dense,
         'complex128_step': c128,
+        'complex64_vs_complex128': c64_err,
+        'marginals_30q': marginals,
         'pauli_observable_30q': pauli,
         'readout_30q': readout,
         'adjoint_24q': adjoint,
@@ -661,7 +832,10 @@ def main():
     }
     if not args.no_cpu:
         print('[bench] CPU baseline ...', file=sys.stderr, flush=True)
-        result['cpu_baseline'] = cpu_baseline(gates, n, amp)
+        cb = cpu_baseline(gates, n, amp)
+        cb.update(host_facts())
+        cb['one_worker'] = cpu_baseline_one_worker()
+        result['cpu_baseline'] = cb
     print(json.dumps(result))
 
 

@@ -314,8 +314,8 @@ int qghip_profile_gate(qghip_qubit_processor *p, qghip_qubit_states *qs, const d
                        const int *local_control_lanes, int n_controls, int local_target_lane,
                        int reps, double *avg_ms);
 
-/* Time `reps` runs of the library's non-temporal streaming copy (16-byte units, four loads
- * in flight per thread) over two fresh device buffers of `bytes` bytes (a multiple of 16),
+/* Time `reps` runs of the library's non-temporal streaming copy (one 16-byte unit per
+ * thread, one wavefront per block, as the fastest gate kernels) over two fresh device buffers of `bytes` bytes (a multiple of 16),
  * HIP events on the library stream; avg_ms receives the mean kernel duration. The copy moves
  * 2 * bytes per run: the HBM streaming ceiling the gate kernels are compared with. */
 int qghip_profile_copy(int64_t bytes, int reps, double *avg_ms);

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "tile_common.hpp"
 #include "qg_runtime.hpp"
 
 namespace qg {
@@ -354,6 +355,102 @@ __global__ __launch_bounds__(kBlock) void k_unitary(Cx<R> *__restrict__ a,
     }
 }
 
+// Dense block staged through LDS: one workgroup per tile of 2^M amplitudes (32 KiB) made of
+// the block's lanes and the lowest other lanes, so the tile is read and written as rows of
+// 2^low contiguous amplitudes in 16-byte units. Every group of 2^K amplitudes of the tile is
+// multiplied by U in LDS (row sums from zero in column order, as k_unitary). With fewer
+// groups than threads, T / G threads share a group and take D / (T / G) rows each, after a
+// barrier that lets every thread read its inputs first.
+template <class R>
+struct DenseTileCfg;
+template <>
+struct DenseTileCfg<float> {
+    static constexpr int M = 12, MaxK = 4;   // complex64 blocks of 5-6 lanes take the matrix cores
+};
+template <>
+struct DenseTileCfg<double> {
+    static constexpr int M = 11, MaxK = 5;   // U of 6 lanes (64 KiB) does not fit beside the tile
+};
+
+struct DenseTileArgs {
+    alignas(4) signed char pos[kTilePosMax];   // state bit of each local tile bit, ascending
+    int low;                                   // local bits 0 .. low-1 are state bits 0 .. low-1
+    unsigned rest;                             // local bits outside the block
+    unsigned fmem[1 << kMaxFuse];              // local offset of matrix index c
+};
+
+template <class R, int K>
+__global__ __launch_bounds__(kBlock) void k_unitary_tile(Cx<R> *__restrict__ a, const Cx<R> *__restrict__ umat,
+                                                         const DenseTileArgs g, u64 first) {
+    constexpr int M = DenseTileCfg<R>::M, D = 1 << K, G = 1 << (M - K), T = kBlock;
+    constexpr int lw = Unit<R>::W == 2 ? 1 : 0;
+    constexpr int S = G >= T ? 1 : T / G;   // threads per group
+    constexpr int RPT = D / S;               // rows per thread
+    __shared__ Cx<R> t[1 << M];
+    __shared__ Cx<R> su[D * D];
+    for (int i = threadIdx.x; i < D * D; i += T) su[i] = umat[i];
+    const u64 base = tile_base(first + blockIdx.x, g.pos, M);
+    tile_to_lds<R, T, (1 << (M - lw)) / T>(t, a, base, g.pos, M, g.low);
+    __syncthreads();
+    for (int q0 = 0; q0 < G; q0 += T / S) {
+        const int q = q0 + (int)threadIdx.x / S, part = (int)threadIdx.x % S;
+        // local index of the group: q deposited at the local bits outside the block
+        unsigned lb = 0, qq = (unsigned)q;
+        for (unsigned m = g.rest; m != 0; m &= m - 1) {
+            if (qq & 1u) lb |= m & (~m + 1);
+            qq >>= 1;
+        }
+        Cx<R> x[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) x[c] = t[lb | g.fmem[c]];
+        if constexpr (S > 1) __syncthreads();
+#pragma unroll 1
+        for (int rr = 0; rr < RPT; ++rr) {
+            const int r = part * RPT + rr;
+            Cx<R> acc = cx<R>(0, 0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) acc = cadd(acc, cmul(su[r * D + c], x[c]));
+            t[lb | g.fmem[r]] = acc;
+        }
+    }
+    __syncthreads();
+    lds_to_tile<R, T, (1 << (M - lw)) / T>(a, t, base, g.pos, M, g.low);
+}
+
+template <class R>
+bool use_dense_tile(int k, int n_lanes) {
+    return k >= 2 && k <= DenseTileCfg<R>::MaxK && n_lanes >= DenseTileCfg<R>::M;
+}
+
+// Tile layout of a block on `lanes` (matrix index bit j <-> lanes[j]): the block's lanes and
+// the lowest other lanes up to M tile bits.
+template <class R>
+DenseTileArgs dense_tile_args(int n_lanes, const std::vector<int> &lanes) {
+    constexpr int M = DenseTileCfg<R>::M;
+    std::vector<int> bits(lanes);
+    for (int b = 0; (int)bits.size() < M && b < n_lanes; ++b)
+        if (std::find(bits.begin(), bits.end(), b) == bits.end()) bits.push_back(b);
+    std::sort(bits.begin(), bits.end());
+    QG_REQUIRE((int)bits.size() == M, "dense tile needs %d lanes", M);
+    DenseTileArgs g;
+    std::memset(&g, 0, sizeof(g));
+    for (int j = 0; j < M; ++j) g.pos[j] = (signed char)bits[j];
+    g.low = 0;
+    while (g.low < M && bits[g.low] == g.low) ++g.low;
+    auto local = [&](int lane) { return (int)(std::find(bits.begin(), bits.end(), lane) - bits.begin()); };
+    unsigned fmask = 0;
+    for (int l : lanes) fmask |= 1u << local(l);
+    g.rest = ((1u << M) - 1) & ~fmask;
+    const int k = (int)lanes.size();
+    for (int c = 0; c < (1 << k); ++c) {
+        unsigned off = 0;
+        for (int j = 0; j < k; ++j)
+            if (c & (1 << j)) off |= 1u << local(lanes[j]);
+        g.fmem[c] = off;
+    }
+    return g;
+}
+
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // Dense 2^K x 2^K unitary on complex64 amplitudes on FP32 matrix cores (K = 5, 6), where the
@@ -515,6 +612,24 @@ void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, co
         }
     }
     const Cx<R> *dm = static_cast<const Cx<R> *>(dmat);
+    if (use_dense_tile<R>(k, n_lanes)) {
+        const DenseTileArgs tg = dense_tile_args<R>(n_lanes, lanes);
+        const u64 n_tiles = u64(1) << (n_lanes - DenseTileCfg<R>::M);
+        const u64 piece = launch_piece_items() / kBlock;
+        for (u64 first = 0; first < n_tiles; first += piece) {
+            const unsigned grid = (unsigned)std::min(piece, n_tiles - first);
+            switch (k) {
+            case 2: hipLaunchKernelGGL((k_unitary_tile<R, 2>), grid, kBlock, 0, s, a, dm, tg, first); break;
+            case 3: hipLaunchKernelGGL((k_unitary_tile<R, 3>), grid, kBlock, 0, s, a, dm, tg, first); break;
+            case 4: hipLaunchKernelGGL((k_unitary_tile<R, 4>), grid, kBlock, 0, s, a, dm, tg, first); break;
+            case 5:
+                if constexpr (DenseTileCfg<R>::MaxK >= 5)
+                    hipLaunchKernelGGL((k_unitary_tile<R, 5>), grid, kBlock, 0, s, a, dm, tg, first);
+                break;
+            }
+        }
+        return;
+    }
     for_each_piece(g.n_groups, [&](u64 first, u64 count) {
         const unsigned grid = grid_for(count, kBlock);
         switch (k) {

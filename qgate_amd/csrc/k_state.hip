@@ -708,24 +708,13 @@ void launch_pauli_decohere(Cx<R> *a, int n_lanes, u64 x, u64 z, int value, doubl
 
 namespace {
 
-// dst = src over n_units 16-byte units, kCopyUnits units per thread with every load in flight
-// before the stores, non-temporal both ways: the streaming ceiling a state pass can reach.
-constexpr int kCopyUnits = 4;
-
-__global__ __launch_bounds__(kBlock) void k_stream_copy(f32x4 *__restrict__ dst, const f32x4 *__restrict__ src,
-                                                        u64 n_units, u64 first) {
-    const u64 base = first + (u64)blockIdx.x * (kBlock * kCopyUnits) + threadIdx.x;
-    f32x4 v[kCopyUnits];
-#pragma unroll
-    for (int r = 0; r < kCopyUnits; ++r) {
-        const u64 u = base + (u64)r * kBlock;
-        if (u < n_units) v[r] = __builtin_nontemporal_load(src + u);
-    }
-#pragma unroll
-    for (int r = 0; r < kCopyUnits; ++r) {
-        const u64 u = base + (u64)r * kBlock;
-        if (u < n_units) __builtin_nontemporal_store(v[r], dst + u);
-    }
+// dst = src over n_units 16-byte units, one unit per thread, one wavefront per block (the
+// shape of the fastest gate kernels), non-temporal both ways: the streaming ceiling a state
+// pass can reach.
+__global__ __launch_bounds__(kWave) void k_stream_copy(f32x4 *__restrict__ dst, const f32x4 *__restrict__ src,
+                                                       u64 n_units, u64 first) {
+    const u64 u = first + (u64)blockIdx.x * kWave + threadIdx.x;
+    if (u < n_units) __builtin_nontemporal_store(__builtin_nontemporal_load(src + u), dst + u);
 }
 
 }  // namespace
@@ -733,13 +722,11 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(f32x4 *__restrict__ dst,
 void launch_stream_copy(void *dst, const void *src, u64 bytes) {
     QG_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
     const u64 n_units = bytes / 16;
-    const u64 per_block = (u64)kBlock * kCopyUnits;
     hipStream_t s = stream();
-    // pieces of whole blocks keep every grid below 2^32 threads
-    const u64 piece = launch_piece_items();
-    for (u64 first = 0; first < n_units; first += piece)
-        hipLaunchKernelGGL(k_stream_copy, (unsigned)((std::min(piece, n_units - first) + per_block - 1) / per_block),
-                           kBlock, 0, s, static_cast<f32x4 *>(dst), static_cast<const f32x4 *>(src), n_units, first);
+    for_each_piece(n_units, [&](u64 first, u64 count) {
+        hipLaunchKernelGGL(k_stream_copy, grid_for(count, kWave), kWave, 0, s, static_cast<f32x4 *>(dst),
+                           static_cast<const f32x4 *>(src), n_units, first);
+    });
     QG_HIP(hipGetLastError());
 }
 

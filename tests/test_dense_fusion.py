@@ -77,6 +77,9 @@ def _apply_numpy(psi, n, lanes, u):
     (10, [9, 0, 4, 6, 2, 7]),   # 16 groups: one matrix-core task
     (9, [8, 0, 4, 6, 2, 7]),    # 8 groups: the vector kernel
     (9, [1, 3, 5, 7, 8]),
+    # several LDS-staged tiles (blocks of 2..4 lanes, complex128 also 5): rows below the lowest
+    # block lane, block lanes above and below bit 6
+    (16, [15, 2]), (16, [13, 6, 0, 9]), (15, [14, 12, 1, 3, 7]), (16, [4, 5, 6]),
 ])
 def test_apply_unitary_matches_numpy(hip_module, dtype, n, lanes):
     rs = np.random.RandomState(len(lanes) * 31 + n)
