@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
@@ -179,91 +180,138 @@ static_assert(kPoolThreads >= kTileChunks, "one thread per chunk for the chunk s
 struct StateSrc {};
 struct ProbSrc {};
 
-// One block per tile: p = |a|^2 (from the state) or p given; chunk sums; serial scan of the
-// 256 chunk sums; tile total; last bin with p > 0.
+// Tile values loaded as 16-byte units: two complex64 amplitudes, one complex128 amplitude or
+// two FP64 probabilities per unit.
+template <class R, bool FROM_STATE>
+struct PoolUnit {
+    static constexpr int V = FROM_STATE ? Unit<R>::W : 2;           // values per unit
+    static constexpr int U = kTile / V / kPoolThreads;              // units per thread
+    typedef typename std::conditional<FROM_STATE, typename UnitVec<R>::T, f64x2>::type T;
+};
+
+// Persistent workgroups over the tiles (stride gridDim.x); each thread issues the next tile's
+// loads before working on the current one, so HBM reads stay in flight through the scan. Per
+// tile: p = |a|^2 (from the state) or p given; chunk sums; fixed-order scan of the 256 chunk
+// sums; the tile total; the last bin with p > 0.
 template <class R, bool FROM_STATE, bool STORE_P>
 __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restrict__ a, i64 n_bins,
                                                       double *__restrict__ p, double *__restrict__ e_inc,
-                                                      double *__restrict__ t_tot, i64 *__restrict__ t_last) {
+                                                      double *__restrict__ t_tot, i64 *__restrict__ t_last,
+                                                      i64 n_tiles) {
+    typedef PoolUnit<R, FROM_STATE> PU;
+    constexpr int V = PU::V, U = PU::U;
+    typedef typename PU::T T;
     __shared__ double sp[kTile + kTile / kChunk];   // one pad slot per chunk
     __shared__ double sc[kTileChunks];
     __shared__ i64 sl[kPoolThreads / 64];
-    const i64 base = (i64)blockIdx.x * kTile;
-    // every load of the thread first (kPoolRows of them in flight), then the FP64 probabilities
-    double v[kPoolRows];
-    if constexpr (FROM_STATE) {
-        Cx<R> x[kPoolRows];
+    const T *src = FROM_STATE ? reinterpret_cast<const T *>(a) : reinterpret_cast<const T *>(p);
+    const i64 n_units = n_bins / V;   // n_bins is a power of two >= 2
+    // Loads are unconditional (units past the end read the last unit, and their values are
+    // replaced by zeros where they are used): a load inside a branch leaves the compiler's
+    // count of loads in flight unknown, and it then waits for all of them before the scan.
+    auto load = [&](i64 tile, T (&raw)[U]) __attribute__((always_inline)) {
+        const i64 ubase = tile * (kTile / V);
 #pragma unroll
-        for (int r = 0; r < kPoolRows; ++r) {
-            const i64 bin = base + r * kPoolThreads + threadIdx.x;
-            x[r] = bin < n_bins ? a[bin] : cx<R>(0, 0);
+        for (int r = 0; r < U; ++r) {
+            const i64 u = ubase + r * kPoolThreads + threadIdx.x;
+            raw[r] = __builtin_nontemporal_load(src + (u < n_units ? u : n_units - 1));
         }
+    };
+    // the per-tile work on loaded values `cur`
+    auto process = [&](i64 tile, const T (&cur)[U]) __attribute__((always_inline)) {
+        const i64 base = tile * kTile;
 #pragma unroll
-        for (int r = 0; r < kPoolRows; ++r) v[r] = abs2d(x[r]);
-    } else {
+        for (int r = 0; r < U; ++r) {
+            const bool in = tile * (kTile / V) + r * kPoolThreads + (i64)threadIdx.x < n_units;
+            double v[V];
+            if constexpr (FROM_STATE) {
+                Amps<R> am;
+                __builtin_memcpy(&am, &cur[r], sizeof(T));
 #pragma unroll
-        for (int r = 0; r < kPoolRows; ++r) {
-            const i64 bin = base + r * kPoolThreads + threadIdx.x;
-            v[r] = bin < n_bins ? p[bin] : 0.;
+                for (int j = 0; j < V; ++j) v[j] = in ? abs2d(am.v[j]) : 0.;
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) v[j] = in ? (double)cur[r][j] : 0.;
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int li = (r * kPoolThreads + threadIdx.x) * V + j;
+                const i64 bin = base + li;
+                // probabilities from the state, and zeros for the padding up to whole tiles
+                if (STORE_P && (FROM_STATE || bin >= n_bins)) p[bin] = v[j];
+                sp[li + li / kChunk] = v[j];
+            }
         }
+        __syncthreads();
+        // serial sum over this thread's chunk (threads below kTileChunks)
+        const int c = threadIdx.x;
+        i64 last = -1;
+        if (c < kTileChunks) {
+            double s = 0.;
+            for (int k = 0; k < kChunk; ++k) {
+                const int li = c * kChunk + k;
+                const double x = sp[li + li / kChunk];
+                s = (k == 0) ? x : s + x;
+                if (x > 0.) last = base + li;
+            }
+            sc[c] = s;
+        }
+        // the tile's last nonzero bin: maximum over the wavefront, then over the waves (below)
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
+        if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = last;
+        __syncthreads();
+        // inclusive scan of the chunk totals by one wavefront, in a fixed order: each lane scans
+        // four consecutive chunks, the lanes' totals are scanned across the wavefront
+        // (inclusive, doubling distances), and each lane adds the total of the lanes before it
+        static_assert(kTileChunks == 4 * 64, "one wavefront scans the chunk totals, four per lane");
+        if (threadIdx.x < 64) {
+            const int l = threadIdx.x;
+            const double q0 = sc[4 * l], q1 = q0 + sc[4 * l + 1], q2 = q1 + sc[4 * l + 2], q3 = q2 + sc[4 * l + 3];
+            double incl = q3;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const double y = __shfl_up(incl, (unsigned)d, 64);
+                if (l >= d) incl += y;
+            }
+            double before = __shfl_up(incl, 1u, 64);
+            if (l == 0) before = 0.;
+            sc[4 * l] = before + q0;
+            sc[4 * l + 1] = before + q1;
+            sc[4 * l + 2] = before + q2;
+            sc[4 * l + 3] = before + q3;
+            // the tile total is its last chunk prefix, so the levels above agree with it exactly
+            if (l == 63) t_tot[tile] = before + q3;
+            if (l == 0) {
+                i64 m = sl[0];
+#pragma unroll
+                for (int w = 1; w < kPoolThreads / 64; ++w) m = max(m, sl[w]);
+                t_last[tile] = m;
+            }
+        }
+        __syncthreads();
+        if (c < kTileChunks) e_inc[tile * kTileChunks + c] = sc[c];
+        // sp and sc are rewritten for the next tile only after every thread passed this point
+        __syncthreads();
+    };
+    // two register buffers used in turn (no copies between them, whose reads would wait for
+    // the loads in flight): the next tile's loads are issued before this tile's stores, and
+    // CDNA retires vector memory operations in issue order
+    // (the prefetch is unconditional too: past the last tile it reads the current one again)
+    const i64 stride = gridDim.x;
+    T bufa[U], bufb[U];
+    i64 tile = blockIdx.x;
+    if (tile < n_tiles) load(tile, bufa);
+    while (tile < n_tiles) {
+        const i64 t1 = tile + stride;
+        load(t1 < n_tiles ? t1 : tile, bufb);
+        process(tile, bufa);
+        if (t1 >= n_tiles) break;
+        const i64 t2 = t1 + stride;
+        load(t2 < n_tiles ? t2 : t1, bufa);
+        process(t1, bufb);
+        tile = t2;
     }
-#pragma unroll
-    for (int r = 0; r < kPoolRows; ++r) {
-        const int li = r * kPoolThreads + threadIdx.x;
-        const i64 bin = base + li;
-        // probabilities from the state, and zeros for the padding up to whole tiles
-        if (STORE_P && (FROM_STATE || bin >= n_bins)) p[bin] = v[r];
-        sp[li + li / kChunk] = v[r];
-    }
-    __syncthreads();
-    // serial sum over this thread's chunk (threads below kTileChunks)
-    const int c = threadIdx.x;
-    i64 last = -1;
-    if (c < kTileChunks) {
-        double s = 0.;
-        for (int k = 0; k < kChunk; ++k) {
-            const int li = c * kChunk + k;
-            const double x = sp[li + li / kChunk];
-            s = (k == 0) ? x : s + x;
-            if (x > 0.) last = base + li;
-        }
-        sc[c] = s;
-    }
-    // the tile's last nonzero bin: maximum over the wavefront, then over the waves (below)
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
-    if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = last;
-    __syncthreads();
-    // inclusive scan of the chunk totals by one wavefront, in a fixed order: each lane scans
-    // four consecutive chunks, the lanes' totals are scanned across the wavefront (inclusive,
-    // doubling distances), and each lane adds the total of the lanes before it
-    static_assert(kTileChunks == 4 * 64, "one wavefront scans the chunk totals, four per lane");
-    if (threadIdx.x < 64) {
-        const int l = threadIdx.x;
-        const double q0 = sc[4 * l], q1 = q0 + sc[4 * l + 1], q2 = q1 + sc[4 * l + 2], q3 = q2 + sc[4 * l + 3];
-        double incl = q3;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const double y = __shfl_up(incl, (unsigned)d, 64);
-            if (l >= d) incl += y;
-        }
-        double before = __shfl_up(incl, 1u, 64);
-        if (l == 0) before = 0.;
-        sc[4 * l] = before + q0;
-        sc[4 * l + 1] = before + q1;
-        sc[4 * l + 2] = before + q2;
-        sc[4 * l + 3] = before + q3;
-        // the tile total is its last chunk prefix, so the levels above agree with it exactly
-        if (l == 63) t_tot[blockIdx.x] = before + q3;
-        if (l == 0) {
-            i64 m = sl[0];
-#pragma unroll
-            for (int w = 1; w < kPoolThreads / 64; ++w) m = max(m, sl[w]);
-            t_last[blockIdx.x] = m;
-        }
-    }
-    __syncthreads();
-    if (c < kTileChunks) e_inc[(i64)blockIdx.x * kTileChunks + c] = sc[c];
 }
 
 // One wave per super-tile: serial inclusive scan of the tile totals in place; super total out.
@@ -431,8 +479,10 @@ template <class R, bool FROM_STATE, bool STORE_P>
 void pool_build(PoolData &pd, const Cx<R> *a) {
     hipStream_t s = stream();
     double *d_total = pd.h_inc + pd.n_supers + 1;
-    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE, STORE_P>), (unsigned)pd.n_tiles, kPoolThreads, 0, s, a, pd.n,
-                       pd.p, pd.e_inc, pd.m_inc, pd.t_last);
+    // four workgroups per compute unit fit the tile's 37 KB of LDS
+    const i64 grid = std::min<i64>(pd.n_tiles, (i64)runtime().n_cus * 4);
+    hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE, STORE_P>), (unsigned)grid, kPoolThreads, 0, s, a, pd.n,
+                       pd.p, pd.e_inc, pd.m_inc, pd.t_last, pd.n_tiles);
     QG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pool_super, (unsigned)pd.n_supers, 64, 0, s, pd.m_inc, pd.n_tiles, pd.h_inc, pd.t_last,
                        pd.s_last);

@@ -187,8 +187,12 @@ __device__ __forceinline__ void apply_form(const Mat2<R> &mm, const Cx<R> &x0, c
         y0 = cx<R>(a * x0.re - b * x1.im, a * x0.im + b * x1.re);
         y1 = cx<R>(d * x1.re - c * x0.im, d * x1.im + c * x0.re);
     } else {
-        y0 = cadd(cmul(mm.m00, x0), cmul(mm.m01, x1));
-        y1 = cadd(cmul(mm.m10, x0), cmul(mm.m11, x1));
+        // one multiply and three FMAs per component (a sum of two complex products written
+        // as such takes two multiplies, two FMAs and an add)
+        y0 = cx<R>(fma_r(mm.m00.re, x0.re, fma_r(-mm.m00.im, x0.im, fma_r(mm.m01.re, x1.re, -mm.m01.im * x1.im))),
+                   fma_r(mm.m00.re, x0.im, fma_r(mm.m00.im, x0.re, fma_r(mm.m01.re, x1.im, mm.m01.im * x1.re))));
+        y1 = cx<R>(fma_r(mm.m10.re, x0.re, fma_r(-mm.m10.im, x0.im, fma_r(mm.m11.re, x1.re, -mm.m11.im * x1.im))),
+                   fma_r(mm.m10.re, x0.im, fma_r(mm.m10.im, x0.re, fma_r(mm.m11.re, x1.im, mm.m11.im * x1.re))));
     }
 }
 
@@ -280,6 +284,38 @@ __device__ __forceinline__ void reg_pair_if(Cx<R> (&v)[A], const Mat2<R> &mm, in
     }
     if constexpr (RB > 3) {
         if (p == 3) reg_pair<R, A, 3, F>(v, mm, 0);
+    }
+}
+
+// Diagonal on register bit P: index j is multiplied by m11 if bit P of j is set, else by m00
+// (PHASE: m00 == 1, only the indices with bit P set), the choice made at compile time.
+template <class R, int A, int P, bool PHASE>
+__device__ __forceinline__ void reg_diag(Cx<R> (&v)[A], const Mat2<R> &mm) {
+    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (((j >> P) & 1) != 0)
+            v[j] = cmul(mm.m11, v[j]);
+        else if constexpr (!PHASE)
+            v[j] = cmul(mm.m00, v[j]);
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    pair_marker<64 + P * 2 + (PHASE ? 1 : 0)>();
+}
+
+template <class R, int A, bool PHASE>
+__device__ __forceinline__ void reg_diag_if(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
+    constexpr int RB = Log2<A>::v;
+    if constexpr (RB > 0) {
+        if (p == 0) reg_diag<R, A, 0, PHASE>(v, mm);
+    }
+    if constexpr (RB > 1) {
+        if (p == 1) reg_diag<R, A, 1, PHASE>(v, mm);
+    }
+    if constexpr (RB > 2) {
+        if (p == 2) reg_diag<R, A, 2, PHASE>(v, mm);
+    }
+    if constexpr (RB > 3) {
+        if (p == 3) reg_diag<R, A, 3, PHASE>(v, mm);
     }
 }
 
@@ -381,11 +417,8 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
             } else if (kind <= kRegDiagPhase) {
                 // target on register position tgt: the factor of index j is a scalar choice
                 if (plain) {
-                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                        constexpr int j = decltype(jc)::value;
-                        v[j] = cmul(csel(((j >> tgt) & 1) != 0, mm.m11, mm.m00), v[j]);
-                        __builtin_amdgcn_sched_barrier(0);
-                    });
+                    if (kind == kRegDiag) reg_diag_if<R, A, false>(v, mm, tgt);
+                    if (kind == kRegDiagPhase) reg_diag_if<R, A, true>(v, mm, tgt);
                 } else {
                     static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
                         constexpr int j = decltype(jc)::value;
