@@ -132,6 +132,70 @@ __global__ __launch_bounds__(kMargThreads) void k_marginal(const Cx<R> *__restri
     }
 }
 
+// The unsliced walk with the kept-upper bits below the hidden-upper ones: a wavefront takes
+// G consecutive kept groups at a time and, for each hidden index, reads all G (adjacent units
+// of one region of the state) before moving to the next hidden index, so each hidden row is
+// read G x 1 KiB at a time instead of 1 KiB. Launched with one block of G groups per
+// wavefront, so all its stores follow its last load (CDNA retires vector memory operations
+// in issue order: a load issued after a store would wait for the store). Each group still
+// sums its hidden indices in ascending order, as k_marginal does, so the results are the
+// same bits.
+template <class R, class O, int G>
+__global__ __launch_bounds__(kMargThreads) void k_marginal_inner(const Cx<R> *__restrict__ a, const MargArgs g,
+                                                                 O *__restrict__ out) {
+    constexpr int W = Unit<R>::W, WB = W == 2 ? 1 : 0;
+    const u64 w = (u64)blockIdx.x * (kMargThreads / 64) + threadIdx.x / 64;
+    if (w >= g.n_waves) return;
+    const int lane = threadIdx.x & 63;
+    const u64 hu_count = u64(1) << g.nh_u;
+    const u64 groups = (u64(1) << g.c) >> g.nh_u;   // per wavefront, a multiple of G
+    u64 grp = (w << g.c) >> g.nh_u;
+    u64 ku_bits = deposit(grp, g.ku_mask);
+    const u64 k_low = extract((u64)lane << WB, g.kl_mask);
+    const bool writer = (lane & g.hl_lane) == 0;
+    for (u64 gb = 0; gb < groups; gb += G) {
+        u64 kb[G];
+        kb[0] = ku_bits;
+#pragma unroll
+        for (int i = 1; i < G; ++i) kb[i] = next_sub(kb[i - 1], g.ku_mask);
+        ku_bits = next_sub(kb[G - 1], g.ku_mask);
+        double s0[G], s1[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) s0[i] = s1[i] = 0.;
+        u64 hb = 0;
+#pragma unroll 2
+        for (u64 h = 0; h < hu_count; ++h) {
+            Amps<R> v[G];
+#pragma unroll
+            for (int i = 0; i < G; ++i) v[i] = load_unit(a, ((kb[i] | hb) >> WB) + (u64)lane);
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                s0[i] += abs2d(v[i].v[0]);
+                if (W == 2) s1[i] += abs2d(v[i].v[W - 1]);
+            }
+            hb = next_sub(hb, g.hu_mask);
+        }
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            double t0 = s0[i], t1 = s1[i];
+            if (W == 2 && g.h_in_unit) t0 = t0 + t1;
+#pragma unroll
+            for (int bit = 0; bit < 6; ++bit) {
+                if ((g.hl_lane >> bit) & 1) {
+                    t0 += __shfl_xor(t0, 1 << bit, 64);
+                    if (W == 2 && !g.h_in_unit) t1 += __shfl_xor(t1, 1 << bit, 64);
+                }
+            }
+            if (writer) {
+                const u64 k = ((grp + i) << g.nk_low) | k_low;
+                out[k] = (O)t0;
+                if (W == 2 && !g.h_in_unit) out[k + 1] = (O)t1;
+            }
+        }
+        grp += G;
+    }
+}
+
 // out[k] = sum over slices s of partial[s * n_out + k], s ascending (few slices: one thread
 // per output), or lanes over s then a butterfly (many slices: one wavefront per output).
 template <class O>
@@ -240,6 +304,21 @@ void launch_marginal(const Cx<R> *a, int n, u64 h_mask, O *out) {
     g.sliced = g.nh_u > c ? 1 : 0;
     g.n_waves = u64(1) << (upper - c);
     const unsigned blocks = (unsigned)((g.n_waves + kMargThreads / 64 - 1) / (kMargThreads / 64));
+    // kept-upper bits below the hidden-upper ones, with at least 2^12 blocks of 8 kept groups:
+    // one block of 8 adjacent groups per wavefront, each hidden row read 8 KiB at a time
+    constexpr int kInnerBits = 3;
+    if (g.nh_u >= 1 && g.nh_u <= 10 && g.ku_mask != 0 &&
+        (g.ku_mask & (~g.ku_mask + 1)) < (g.hu_mask & (~g.hu_mask + 1)) &&
+        upper - g.nh_u - kInnerBits >= 12) {
+        MargArgs gi = g;
+        gi.c = g.nh_u + kInnerBits;
+        gi.sliced = 0;
+        gi.n_waves = u64(1) << (upper - gi.c);
+        const unsigned iblocks = (unsigned)((gi.n_waves + kMargThreads / 64 - 1) / (kMargThreads / 64));
+        hipLaunchKernelGGL((k_marginal_inner<R, O, 1 << kInnerBits>), iblocks, kMargThreads, 0, s, a, gi, out);
+        QG_HIP(hipGetLastError());
+        return;
+    }
     if (!g.sliced) {
         hipLaunchKernelGGL((k_marginal<R, O>), blocks, kMargThreads, 0, s, a, g, out, nullptr);
         QG_HIP(hipGetLastError());
