@@ -369,7 +369,7 @@ struct DenseTileCfg<float> {
 };
 template <>
 struct DenseTileCfg<double> {
-    static constexpr int M = 11, MaxK = 5;   // U of 6 lanes (64 KiB) does not fit beside the tile
+    static constexpr int M = 11, MaxK = 4;   // 5 and 6 lanes take the FP64 matrix cores
 };
 
 struct DenseTileArgs {
@@ -452,65 +452,88 @@ DenseTileArgs dense_tile_args(int n_lanes, const std::vector<int> &lanes) {
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Dense 2^K x 2^K unitary on complex64 amplitudes on FP32 matrix cores (K = 5, 6), where the
-// product's 8 x 2^K flops per amplitude pass the vector units' rate. y = U x is the real
-// product [yr; yi] = W [xr; xi], W = [[Ur, -Ui], [Ui, Ur]] (M x M, M = 2^(K+1)), with one
-// column per group of 2^K amplitudes. A wavefront takes 16 groups at a time and accumulates
-// the M / 16 output blocks of 16 rows x 16 groups with v_mfma_f32_16x16x4f32, four input rows
-// per instruction:
+// 16 x 16 x 4 matrix-core step in the precision of R: acc += A (16 x 4) B (4 x 16).
+template <class R>
+struct Mfma;
+// Both take A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k. Result element v of
+// lane l is D[row][l % 16] with row = 4 (l / 16) + v for FP32 and row = l / 16 + 4 v for FP64
+// (measured on gfx950: one step on known A and B, every lane's four results identified).
+template <>
+struct Mfma<float> {
+    typedef v4f Acc;
+    static __device__ __forceinline__ Acc step(float a, float b, Acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static constexpr int Pad = 16;        // column stride M + 16 floats: the four fragments on distinct banks
+    static constexpr bool RowStep4 = false;   // element v: row 4 (l / 16) + v
+};
+template <>
+struct Mfma<double> {
+    typedef v4d Acc;
+    static __device__ __forceinline__ Acc step(double a, double b, Acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static constexpr int Pad = 8;         // 64-byte column offset, as for FP32
+    static constexpr bool RowStep4 = true;    // element v: row l / 16 + 4 v
+};
+
+// Dense 2^K x 2^K unitary on matrix cores (K = 5, 6; FP32 for complex64, FP64 for
+// complex128), where the product's 8 x 2^K flops per amplitude pass the vector units' rate.
+// y = U x is the real product [yr; yi] = W [xr; xi], W = [[Ur, -Ui], [Ui, Ur]] (M x M,
+// M = 2^(K+1)), with one column per group of 2^K amplitudes. A wavefront takes 16 groups at a
+// time and accumulates the M / 16 output blocks of 16 rows x 16 groups with 16 x 16 x 4
+// matrix-core steps, four input rows per step:
 //   A operand, lane l: W[16 t + l % 16][4 kk + l / 16] from LDS (W stored column-major with
 //     a padded column stride, so the four 16-lane fragments of a read fall on distinct banks);
 //   B operand, lane l: row 4 kk + l / 16 of group l % 16, from registers (lane l loads the
 //     amplitudes c = l / 16 (mod 4) of its group);
-//   result element v of lane l: row 16 t + 4 (l / 16) + v of group l % 16; blocks t and
-//     t + M / 32 hold the real and imaginary part of the same amplitude.
+//   result element v of lane l: row 16 t + 4 (l / 16) + v (FP32) or 16 t + l / 16 + 4 v
+//     (FP64) of group l % 16; blocks t and t + M / 32 hold the real and imaginary part of the
+//     same amplitude.
 // Persistent workgroups stride over the tasks, so W is staged into LDS once per workgroup.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<float> *__restrict__ a,
-                                                         const float *__restrict__ wcol,
-                                                         const UnitaryArgs<float> g, u64 n_tasks) {
-    constexpr int D = 1 << K, M = 2 * D, NT = M / 16, MS = M + 16;
-    __shared__ float w[M * MS];   // w[c * MS + r] = W[r][c]
+template <class R, int K>
+__global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<R> *__restrict__ a, const R *__restrict__ wcol,
+                                                         const UnitaryArgs<R> g, u64 n_tasks) {
+    typedef typename Mfma<R>::Acc Acc;
+    constexpr int D = 1 << K, M = 2 * D, NT = M / 16, MS = M + Mfma<R>::Pad;
+    __shared__ R w[M * MS];   // w[c * MS + r] = W[r][c]
     for (int i = threadIdx.x; i < M * M; i += kBlock) w[(i / M) * MS + i % M] = wcol[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, p = lane >> 4;
     const u64 stride = (u64)gridDim.x * (kBlock / 64);
-    // member offsets are ORs of per-bit offsets: member[4 q + p] = member[4 q] | member[p] and
-    // member[16 t + 4 p + v] = member[16 t + v] | member[4 p], so each lane holds its own part
-    // (two selects, no branches) and ORs in a uniform part per access
+    // member offsets are ORs of per-bit offsets: member[4 q + p] = member[4 q] | member[p], and
+    // an output row is member[16 t + v] | member[4 p] (FP32) or member[16 t + 4 v] | member[p]
+    // (FP64), so each lane holds its own part (two selects, no branches) and ORs in a uniform
+    // part per access
     const u64 lane_in = ((p & 1) ? g.member[1] : 0) | ((p & 2) ? g.member[2] : 0);
-    const u64 lane_out = ((p & 1) ? g.member[4] : 0) | ((p & 2) ? g.member[8] : 0);
+    const u64 lane_out = Mfma<R>::RowStep4 ? lane_in : (((p & 1) ? g.member[4] : 0) | ((p & 2) ? g.member[8] : 0));
     // x_c (c = 4 q + p) of group j of a task
-    auto load = [&](u64 task, float (&xr)[D / 4], float (&xi)[D / 4]) __attribute__((always_inline)) {
+    auto load = [&](u64 task, R (&xr)[D / 4], R (&xi)[D / 4]) __attribute__((always_inline)) {
         const u64 base = insert_zero_bits(task * 16 + j, g.ins) | lane_in;
 #pragma unroll
         for (int q = 0; q < D / 4; ++q) {
-            const Cx<float> v = a[base | g.member[4 * q]];
+            const Cx<R> v = a[base | g.member[4 * q]];
             xr[q] = v.re;
             xi[q] = v.im;
         }
     };
-    u64 task = (u64)blockIdx.x * (kBlock / 64) + wave;
-    float xr[D / 4], xi[D / 4];
-    if (task < n_tasks) load(task, xr, xi);
-    for (; task < n_tasks; task += stride) {
-        // the next task's inputs are in flight while this task's products run
-        float nr[D / 4], ni[D / 4];
-        if (task + stride < n_tasks) load(task + stride, nr, ni);
+    // products and stores of one task, from its loaded inputs
+    auto run = [&](u64 task, const R (&xr)[D / 4], const R (&xi)[D / 4]) __attribute__((always_inline)) {
         const u64 base = insert_zero_bits(task * 16 + j, g.ins) | lane_out;
-        v4f acc[NT];
+        Acc acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[t][v] = 0.f;
+            for (int v = 0; v < 4; ++v) acc[t][v] = R(0);
 #pragma unroll
         for (int kk = 0; kk < M / 4; ++kk) {
-            const float b = kk < D / 4 ? xr[kk] : xi[kk - D / 4];
-            const float *col = w + (4 * kk + p) * MS + j;
+            const R b = kk < D / 4 ? xr[kk] : xi[kk - D / 4];
+            const R *col = w + (4 * kk + p) * MS + j;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(col[16 * t], b, acc[t], 0, 0, 0);
+            for (int t = 0; t < NT; ++t) acc[t] = Mfma<R>::step(col[16 * t], b, acc[t]);
             // one step's matrix reads in flight at a time (hoisting them all would take every
             // register of the wave)
             __builtin_amdgcn_sched_barrier(0);
@@ -519,24 +542,40 @@ __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<float> *__restrict__
         for (int t = 0; t < NT / 2; ++t)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                a[base | g.member[16 * t + v]] = cx<float>(acc[t][v], acc[t + NT / 2][v]);
+                a[base | g.member[16 * t + (Mfma<R>::RowStep4 ? 4 * v : v)]] = cx<R>(acc[t][v], acc[t + NT / 2][v]);
             }
-#pragma unroll
-        for (int q = 0; q < D / 4; ++q) {
-            xr[q] = nr[q];
-            xi[q] = ni[q];
-        }
+    };
+    // Two input buffers used in turn, the next task's loads issued before this task's products
+    // and stores, unconditionally (past the last task they read this task again): CDNA retires
+    // vector memory operations in issue order, and a copy between buffers or a load inside a
+    // branch would make the compiler wait for every access in flight, this task's stores too.
+    u64 task = (u64)blockIdx.x * (kBlock / 64) + wave;
+    R ar[D / 4], ai[D / 4], br[D / 4], bi[D / 4];
+    if (task < n_tasks) load(task, ar, ai);
+    while (task < n_tasks) {
+        const u64 t1 = task + stride;
+        load(t1 < n_tasks ? t1 : task, br, bi);
+        run(task, ar, ai);
+        if (t1 >= n_tasks) break;
+        const u64 t2 = t1 + stride;
+        load(t2 < n_tasks ? t2 : t1, ar, ai);
+        run(t1, br, bi);
+        task = t2;
     }
 }
 
-// Workgroups per compute unit of the matrix-core kernel (LDS: 20 KiB for K = 5, 72 KiB for 6).
-constexpr int mfma_groups_per_cu(int k) { return k == 6 ? 2 : 8; }
+// Workgroups per compute unit of the matrix-core kernel (LDS of W: complex64 20 KiB for K = 5,
+// 72 KiB for 6; complex128 36 KiB and 136 KiB).
+template <class R>
+constexpr int mfma_groups_per_cu(int k) {
+    return sizeof(R) == 4 ? (k == 6 ? 2 : 8) : (k == 6 ? 1 : 4);
+}
 
-// The matrix-core kernel takes complex64 blocks of 5 or 6 qubits on states of at least 16
-// groups; everything else takes the vector kernel.
+// The matrix-core kernel takes blocks of 5 or 6 qubits on states of at least 16 groups;
+// everything else takes the vector kernels.
 template <class R>
 bool use_mfma(int k, int n_lanes) {
-    return std::is_same<R, float>::value && (k == 5 || k == 6) && n_lanes - k >= 4;
+    return (k == 5 || k == 6) && n_lanes - k >= 4;
 }
 
 // Device matrix of a block: complex row-major for the vector kernel (D x D Cx<R>), the real
@@ -544,7 +583,7 @@ bool use_mfma(int k, int n_lanes) {
 template <class R>
 size_t unitary_bytes(int k, int n_lanes) {
     const size_t D = size_t(1) << k;
-    return use_mfma<R>(k, n_lanes) ? sizeof(float) * 4 * D * D : sizeof(Cx<R>) * D * D;
+    return use_mfma<R>(k, n_lanes) ? sizeof(R) * 4 * D * D : sizeof(Cx<R>) * D * D;
 }
 
 template <class R>
@@ -552,11 +591,11 @@ void pack_unitary(int k, int n_lanes, const std::vector<std::complex<double>> &m
     const int D = 1 << k;
     if (use_mfma<R>(k, n_lanes)) {
         const int M = 2 * D;
-        float *w = reinterpret_cast<float *>(out);
+        R *w = reinterpret_cast<R *>(out);
         for (int r = 0; r < M; ++r)
             for (int c = 0; c < M; ++c) {
                 const std::complex<double> u = mat[(size_t)(r % D) * D + (c % D)];
-                const float re = (float)u.real(), im = (float)u.imag();
+                const R re = (R)u.real(), im = (R)u.imag();
                 w[(size_t)c * M + r] = (r < D) == (c < D) ? re : (r < D ? -im : im);
             }
         return;
@@ -597,19 +636,17 @@ void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, co
     const UnitaryArgs<R> g = unitary_args<R>(n_lanes, lanes);
     const int k = g.k;
     hipStream_t s = stream();
-    if constexpr (std::is_same<R, float>::value) {
-        if (use_mfma<R>(k, n_lanes)) {
-            const u64 n_tasks = g.n_groups / 16;
-            const int per_cu = mfma_groups_per_cu(k);
-            const unsigned grid =
-                (unsigned)std::min<u64>((n_tasks + 3) / 4, (u64)std::max(1, runtime().n_cus * per_cu));
-            const float *w = static_cast<const float *>(dmat);
-            if (k == 5)
-                hipLaunchKernelGGL((k_unitary_mfma<5>), grid, kBlock, 0, s, a, w, g, n_tasks);
-            else
-                hipLaunchKernelGGL((k_unitary_mfma<6>), grid, kBlock, 0, s, a, w, g, n_tasks);
-            return;
-        }
+    if (use_mfma<R>(k, n_lanes)) {
+        const u64 n_tasks = g.n_groups / 16;
+        const int per_cu = mfma_groups_per_cu<R>(k);
+        const unsigned grid =
+            (unsigned)std::min<u64>((n_tasks + 3) / 4, (u64)std::max(1, runtime().n_cus * per_cu));
+        const R *w = static_cast<const R *>(dmat);
+        if (k == 5)
+            hipLaunchKernelGGL((k_unitary_mfma<R, 5>), grid, kBlock, 0, s, a, w, g, n_tasks);
+        else
+            hipLaunchKernelGGL((k_unitary_mfma<R, 6>), grid, kBlock, 0, s, a, w, g, n_tasks);
+        return;
     }
     const Cx<R> *dm = static_cast<const Cx<R> *>(dmat);
     if (use_dense_tile<R>(k, n_lanes)) {
@@ -622,10 +659,6 @@ void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, co
             case 2: hipLaunchKernelGGL((k_unitary_tile<R, 2>), grid, kBlock, 0, s, a, dm, tg, first); break;
             case 3: hipLaunchKernelGGL((k_unitary_tile<R, 3>), grid, kBlock, 0, s, a, dm, tg, first); break;
             case 4: hipLaunchKernelGGL((k_unitary_tile<R, 4>), grid, kBlock, 0, s, a, dm, tg, first); break;
-            case 5:
-                if constexpr (DenseTileCfg<R>::MaxK >= 5)
-                    hipLaunchKernelGGL((k_unitary_tile<R, 5>), grid, kBlock, 0, s, a, dm, tg, first);
-                break;
             }
         }
         return;

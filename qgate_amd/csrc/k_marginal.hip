@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kMargThreads) void k_marginal(const Cx<R> *__restri
 // The unsliced walk with the kept-upper bits below the hidden-upper ones: a wavefront takes
 // G consecutive kept groups at a time and, for each hidden index, reads all G (adjacent units
 // of one region of the state) before moving to the next hidden index, so each hidden row is
-// read G x 1 KiB at a time instead of 1 KiB. Launched with one block of G groups per
+// read G x 1 KiB at a time instead of 1 KiB (G = 16). Launched with one block of G groups per
 // wavefront, so all its stores follow its last load (CDNA retires vector memory operations
 // in issue order: a load issued after a store would wait for the store). Each group still
 // sums its hidden indices in ascending order, as k_marginal does, so the results are the
@@ -304,9 +304,9 @@ void launch_marginal(const Cx<R> *a, int n, u64 h_mask, O *out) {
     g.sliced = g.nh_u > c ? 1 : 0;
     g.n_waves = u64(1) << (upper - c);
     const unsigned blocks = (unsigned)((g.n_waves + kMargThreads / 64 - 1) / (kMargThreads / 64));
-    // kept-upper bits below the hidden-upper ones, with at least 2^12 blocks of 8 kept groups:
-    // one block of 8 adjacent groups per wavefront, each hidden row read 8 KiB at a time
-    constexpr int kInnerBits = 3;
+    // kept-upper bits below the hidden-upper ones, with at least 2^12 blocks of 16 kept groups:
+    // one block of 16 adjacent groups per wavefront, each hidden row read 16 KiB at a time
+    constexpr int kInnerBits = 4;
     if (g.nh_u >= 1 && g.nh_u <= 10 && g.ku_mask != 0 &&
         (g.ku_mask & (~g.ku_mask + 1)) < (g.hu_mask & (~g.hu_mask + 1)) &&
         upper - g.nh_u - kInnerBits >= 12) {

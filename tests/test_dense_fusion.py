@@ -80,6 +80,8 @@ def _apply_numpy(psi, n, lanes, u):
     # several LDS-staged tiles (blocks of 2..4 lanes, complex128 also 5): rows below the lowest
     # block lane, block lanes above and below bit 6
     (16, [15, 2]), (16, [13, 6, 0, 9]), (15, [14, 12, 1, 3, 7]), (16, [4, 5, 6]),
+    # matrix cores over many tasks (complex128 on the FP64 matrix cores too)
+    (16, [15, 2, 9, 4, 11, 0]), (17, [16, 3, 8, 12, 1]),
 ])
 def test_apply_unitary_matches_numpy(hip_module, dtype, n, lanes):
     rs = np.random.RandomState(len(lanes) * 31 + n)

@@ -320,3 +320,40 @@ def test_batch4096_every_item_against_single_circuits(hip_module):
         c.delete()
     assert worst_v < 2e-5 and worst_g < 2e-4
     assert worst64_v < 1e-4 and worst64_g < 1e-4
+
+
+# ---- complex64 error against the FP64 oracle, per workload ---------------------------------
+
+def test_complex64_error_vs_oracle_per_config(hip_module, port_module):
+    """The complex64 path against the FP64 oracle (the reference CPU runtime restated) on the
+    same circuits: the 16-qubit, 20-layer brickwork of the CPU workload at full size, and the
+    2000-gate random circuit of the gate benchmark at 22 qubits (the largest the oracle runs in
+    seconds) with fusion off, in tile passes and in 6-qubit dense blocks. Max |amplitude error|
+    (north_star bar 5e-6) and max |P(qubit = 0) error| are printed per case."""
+    cases = [('brickwork16x20', lambda: workloads.brickwork(16, 20, seed=1), [('off', 0)])]
+    q22 = new_qregs(22)
+    cases.append(('random22x2000', lambda: (q22, workloads.to_ops(workloads.random_gate_list(22, 2000, 2), q22)),
+                  [('off', 0), ('tile', 0), ('dense', 6)]))
+    for name, make, fusions in cases:
+        q, ops = make()
+        ref = Simulator(port_module, circuit_prep='one_static', dtype=np.float64)
+        ref.qubits.set_ordering(list(q))
+        ref.run(ops)
+        want = np.asarray(ref.qubits.states[:], np.complex128)
+        want_p0 = np.array([ref.qubits.calc_probability(x) for x in q])
+        ref.terminate()
+        for fusion in fusions:
+            hip_module.set_fusion(*fusion)
+            sim = Simulator(hip_module, circuit_prep='one_static', dtype=np.float32)
+            sim.qubits.set_ordering(list(q))
+            try:
+                sim.run(ops)
+                got = np.asarray(sim.qubits.states[:], np.complex128)
+                p0 = np.array([sim.qubits.calc_probability(x) for x in q])
+            finally:
+                _close_sim(sim, hip_module)
+            da = float(np.max(np.abs(got - want)))
+            dp = float(np.max(np.abs(p0 - want_p0)))
+            print('[error] {} complex64 fusion {}: max |amp - FP64 oracle| = {:.3g} (max |amp| {:.3g}), '
+                  'max |P0 - oracle| = {:.3g}'.format(name, fusion, da, float(np.max(np.abs(want))), dp), flush=True)
+            assert da < 5e-6 and dp < 5e-5, (name, fusion, da, dp)
