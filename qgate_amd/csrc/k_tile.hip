@@ -419,13 +419,59 @@ struct SweepCfg<double> {
 
 }  // namespace
 
+// Relative register-pass cost of a 2x2 by its form: diagonal or swap (moves, or one factor
+// per thread when the bit is not a register bit), real, general.
+static int form_cost(const GateMatrix &g) {
+    if (is_diagonal(g)) return 1;
+    if (g.m[0][0] == 0. && g.m[1][1] == 0. && g.m[0][1] == 1. && g.m[1][0] == 1.) return 1;
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+            if (g.m[r][c].imag() != 0.) return 4;
+    return 2;
+}
+
+// Runs of uncontrolled gates on one lane, as one gate each: a gate without controls joins the
+// open run of its lane (product in FP64, rounded once where the pass takes it) when the
+// product's form costs no more than the two apart (a real and a diagonal gate make a general
+// one, so they stay apart), and a gate with controls closes the runs of every lane it touches
+// before it is placed. A run's later gates move ahead of gates on other lanes only, which
+// commute with them.
+static std::vector<QueuedGate> merge_lane_runs(const std::vector<QueuedGate> &gates, int n_lanes) {
+    std::vector<QueuedGate> out;
+    out.reserve(gates.size());
+    std::vector<int> open(n_lanes, -1);   // index in out of the lane's open run
+    for (const QueuedGate &g : gates) {
+        if (g.ctrl.empty()) {
+            const int k = open[g.target];
+            if (k >= 0) {
+                const GateMatrix &m = out[k].m;
+                GateMatrix p;
+                for (int r = 0; r < 2; ++r)
+                    for (int c = 0; c < 2; ++c) p.m[r][c] = g.m.m[r][0] * m.m[0][c] + g.m.m[r][1] * m.m[1][c];
+                if (form_cost(p) <= form_cost(m) + form_cost(g.m)) {
+                    out[k].m = p;
+                    continue;
+                }
+            }
+            open[g.target] = (int)out.size();
+            out.push_back(g);
+            continue;
+        }
+        open[g.target] = -1;
+        for (int c : g.ctrl) open[c] = -1;
+        out.push_back(g);
+    }
+    return out;
+}
+
 template <class R>
-void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int max_bits,
+void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates_in, int max_bits,
                          int low_bits) {
     typedef TileCfg<R> C;
     const int mb = std::min(max_bits > 0 ? std::min(max_bits, C::MaxBits) : C::MaxBits, n_lanes);
     const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
     QG_REQUIRE(Unit<R>::W == 1 || low >= 1, "complex64 tiles need bit 0");
+    const std::vector<QueuedGate> gates = merge_lane_runs(gates_in, n_lanes);
     PassPlan plan;
     auto open = [&]() {
         plan.bits.clear();

@@ -177,3 +177,41 @@ def test_register_and_lds_tiles(hip_module, port_module, dtype, registers):
         hip_module.set_register_tiles(True)
     print('registers', registers, dtype.__name__, err)
     assert err < tol(dtype), err
+
+
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('shape', [(0, -1), (6, 2)])
+def test_single_lane_runs_merged(hip_module, port_module, dtype, shape):
+    """Tile passes fold runs of uncontrolled gates on one lane into one 2x2 (FP64 product) and
+    keep a real and a diagonal gate apart. Runs here are long and concentrated on three lanes
+    (one below the contiguous tile bits, one in the middle, the top one): products that come
+    back to the identity (H H, X X), diagonal (RZ S T U1), real (RY H) and general ones, cut
+    by controlled gates whose controls or targets are those lanes."""
+    from qgate_amd.model import ir
+    n = 13
+    rs = np.random.RandomState(11 + shape[0])
+    q = new_qregs(n)
+    hot = [0, 6, n - 1]
+    one = [(gtype.H, 0), (gtype.X, 0), (gtype.S, 0), (gtype.T, 0), (gtype.Z, 0), (gtype.RZ, 1),
+           (gtype.U1, 1), (gtype.RY, 1), (gtype.RX, 1), (gtype.U, 3), (gtype.SH, 0)]
+    ops = [H(x) for x in q]
+    for _ in range(600):
+        if rs.randint(8) == 0:
+            h = hot[rs.randint(3)]
+            other = int(rs.choice([x for x in range(n) if x != h]))
+            lanes = [h, other] if rs.randint(2) else [other, h]   # [target, control]
+            g = ir.Gate(gtype.X())
+            g.set_qreg(q[lanes[0]])
+            g.set_ctrllist([q[lanes[1]]])
+        else:
+            cls, n_args = one[rs.randint(len(one))]
+            g = ir.Gate(cls(*[float(v) for v in rs.uniform(0, 6.3, n_args)]))
+            g.set_adjoint(bool(rs.randint(2)))
+            g.set_qreg(q[hot[rs.randint(3)]])
+        ops.append(g)
+    ref = _sim(port_module, ops, q, 'one_static', np.float64)
+    with _Fusion(hip_module, 'tile', *shape):
+        got = _sim(hip_module, ops, q, 'one_static', dtype)
+        err = max_abs(got.qubits.states[:], ref.qubits.states[:])
+    print(shape, dtype.__name__, err)
+    assert err < tol(dtype), err
