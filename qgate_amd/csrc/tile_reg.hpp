@@ -196,167 +196,216 @@ __device__ __forceinline__ void apply_form(const Mat2<R> &mm, const Cx<R> &x0, c
     }
 }
 
-// Pair update on register bit P (form F, see apply_form) of the register indices j whose
-// register-position controls creg are set.
-template <class R, int A, int P, int F>
-__device__ __forceinline__ void reg_pair(Cx<R> (&v)[A], const Mat2<R> &mm, int creg) {
+// ---- register ops between two register arrays ----------------------------------------------
+//
+// The ops of a phase read one register array and write the other, which the next op reads;
+// a uniform flag says which array holds the amplitudes. Every output of an op is a new
+// value, so all dispatch arms of an op leave their results in the same registers and no join
+// needs copies. (An in-place update of one loop-carried array made the compiler keep two
+// copies of it anyway and move the whole array between them on most paths: 36 to 130 moves
+// a gate, against 128 FMAs for a general 2x2.) Ops that change no amplitude (skipped for
+// controls outside the tile, per-thread factors) leave the flag as it is and move nothing.
+
+// Arm codes of an op: pairs on register bit P of form F (0 general, 1 swap, 2 real) at
+// 4 F + P, with controls on lane, wave or register bits at 12 + 4 F + P; diagonals on
+// register bit P at 24 + P, with m00 == 1 at 28 + P; a diagonal on a register bit with
+// controls 32; a per-thread factor on the register indices with controls creg 33.
+enum : int { kArmPairSel = 12, kArmDiag = 24, kArmDiagPhase = 28, kArmDiagCtrl = 32, kArmFactorReg = 33 };
+
+// Ends an arm. The markers differ per arm and per array, so the optimiser merges no two arms
+// into one block that picks register elements through pointers (which would move them to
+// scratch memory) or through selects.
+template <int H, int C>
+__device__ __forceinline__ void arm_marker() {
+    asm volatile("; register op %0" ::"n"(H * 64 + C));
+}
+
+// y = x with the pair update of form F on register bit P applied to every pair (plain), or
+// to the register indices j with (j & creg) == creg in threads with ok set (SEL: selects).
+template <class R, int A, int P, int F, bool SEL>
+__device__ __forceinline__ void op_pair(const Cx<R> (&x)[A], Cx<R> (&y)[A], const Mat2<R> &mm, int creg, bool ok) {
     static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
         if constexpr (((j >> P) & 1) == 0) {
             constexpr int j1 = j + (1 << P);
-            if ((j & creg) == creg) {
-                const Cx<R> x0 = v[j], x1 = v[j1];
-                apply_form<R, F>(mm, x0, x1, v[j], v[j1]);
+            if constexpr (SEL) {
+                Cx<R> y0, y1;
+                apply_form<R, F>(mm, x[j], x[j1], y0, y1);
+                const bool c = ok && (j & creg) == creg;
+                y[j] = csel(c, y0, x[j]);
+                y[j1] = csel(c, y1, x[j1]);
+            } else {
+                apply_form<R, F>(mm, x[j], x[j1], y[j], y[j1]);
             }
             __builtin_amdgcn_sched_barrier(0);   // one pair at a time: bounded register use
         }
     });
-    pair_marker<P * 4 + F>();
 }
 
-// Pair update with controls: only register indices with (j & creg) == creg (scalar tests)
-// in threads with ok set change; written as selects, so no masked region needs copies of
-// the registers.
-template <class R, int A, int P, int F>
-__device__ __forceinline__ void reg_pair_sel(Cx<R> (&v)[A], const Mat2<R> &mm, int creg, bool ok) {
-    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-        constexpr int j = decltype(jc)::value;
-        if constexpr (((j >> P) & 1) == 0) {
-            constexpr int j1 = j + (1 << P);
-            const bool c = ok && (j & creg) == creg;
-            const Cx<R> x0 = v[j], x1 = v[j1];
-            Cx<R> y0, y1;
-            apply_form<R, F>(mm, x0, x1, y0, y1);
-            v[j] = csel(c, y0, x0);
-            v[j1] = csel(c, y1, x1);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    });
-    pair_marker<32 + P * 4 + F>();
-}
-
-// Dispatch on the register position p (< RB, guaranteed by the planner) as one switch with
-// an unreachable default: every path ends at one join. A chain of one-armed ifs would give
-// each not-taken arm a copy of the whole register array.
-template <class R, int A, int F>
-__device__ __forceinline__ void reg_pair_sel_at(Cx<R> (&v)[A], const Mat2<R> &mm, int creg, bool ok, int p) {
-    constexpr int RB = Log2<A>::v;
-    switch (p) {
-        case 0:
-            if constexpr (RB > 0) {
-                reg_pair_sel<R, A, 0, F>(v, mm, creg, ok);
-                break;
-            }
-            [[fallthrough]];
-        case 1:
-            if constexpr (RB > 1) {
-                reg_pair_sel<R, A, 1, F>(v, mm, creg, ok);
-                break;
-            }
-            [[fallthrough]];
-        case 2:
-            if constexpr (RB > 2) {
-                reg_pair_sel<R, A, 2, F>(v, mm, creg, ok);
-                break;
-            }
-            [[fallthrough]];
-        case 3:
-            if constexpr (RB > 3) {
-                reg_pair_sel<R, A, 3, F>(v, mm, creg, ok);
-                break;
-            }
-            [[fallthrough]];
-        default: __builtin_unreachable();
-    }
-}
-
-template <class R, int A, int F>
-__device__ __forceinline__ void reg_pair_if(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
-    constexpr int RB = Log2<A>::v;
-    if constexpr (RB > 0) {
-        if (p == 0) reg_pair<R, A, 0, F>(v, mm, 0);
-    }
-    if constexpr (RB > 1) {
-        if (p == 1) reg_pair<R, A, 1, F>(v, mm, 0);
-    }
-    if constexpr (RB > 2) {
-        if (p == 2) reg_pair<R, A, 2, F>(v, mm, 0);
-    }
-    if constexpr (RB > 3) {
-        if (p == 3) reg_pair<R, A, 3, F>(v, mm, 0);
-    }
-}
-
-// Diagonal on register bit P: index j is multiplied by m11 if bit P of j is set, else by m00
-// (PHASE: m00 == 1, only the indices with bit P set), the choice made at compile time.
+// y = x times m11 where bit P of the register index is set, else m00 (PHASE: m00 == 1, those
+// indices copied), the choice made at compile time.
 template <class R, int A, int P, bool PHASE>
-__device__ __forceinline__ void reg_diag(Cx<R> (&v)[A], const Mat2<R> &mm) {
+__device__ __forceinline__ void op_diag(const Cx<R> (&x)[A], Cx<R> (&y)[A], const Mat2<R> &mm) {
     static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
         constexpr int j = decltype(jc)::value;
         if constexpr (((j >> P) & 1) != 0)
-            v[j] = cmul(mm.m11, v[j]);
-        else if constexpr (!PHASE)
-            v[j] = cmul(mm.m00, v[j]);
+            y[j] = cmul(mm.m11, x[j]);
+        else if constexpr (PHASE)
+            y[j] = x[j];
+        else
+            y[j] = cmul(mm.m00, x[j]);
         __builtin_amdgcn_sched_barrier(0);
     });
-    pair_marker<64 + P * 2 + (PHASE ? 1 : 0)>();
 }
 
-template <class R, int A, bool PHASE>
-__device__ __forceinline__ void reg_diag_if(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
+template <class R, int A, int H, int C>
+__device__ __forceinline__ void arm_body(const Cx<R> (&x)[A], Cx<R> (&y)[A], const Mat2<R> &mm, int creg, bool ok,
+                                         int tgt, const Cx<R> &f) {
     constexpr int RB = Log2<A>::v;
-    if constexpr (RB > 0) {
-        if (p == 0) reg_diag<R, A, 0, PHASE>(v, mm);
+    // Complex64: each arm takes its own copy of the matrix entries (scalar registers, through
+    // an empty asm statement). A product of an entry and an amplitude that two arms share
+    // would otherwise be computed once above their branch and be live through either arm,
+    // which raised the kernel above 128 VGPRs (3 workgroups per CU instead of 4).
+    const Cx<R>(&xl)[A] = x;
+    Mat2<R> ml = mm;
+    if constexpr (sizeof(R) == 4)
+        asm volatile("; register op matrix %8"
+                     : "+s"(ml.m00.re), "+s"(ml.m00.im), "+s"(ml.m01.re), "+s"(ml.m01.im), "+s"(ml.m10.re),
+                       "+s"(ml.m10.im), "+s"(ml.m11.re), "+s"(ml.m11.im)
+                     : "n"(H * 64 + C));
+    if constexpr (C < kArmDiag) {
+        constexpr int F = (C % kArmPairSel) / 4, P = C % 4;
+        if constexpr (P < RB)
+            op_pair<R, A, P, F, (C >= kArmPairSel)>(xl, y, ml, creg, ok);
+        else
+            __builtin_unreachable();
+    } else if constexpr (C < kArmDiagCtrl) {
+        constexpr int P = C % 4;
+        if constexpr (P < RB)
+            op_diag<R, A, P, (C >= kArmDiagPhase)>(xl, y, ml);
+        else
+            __builtin_unreachable();
+    } else if constexpr (C == kArmDiagCtrl) {
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const Cx<R> u = cmul(csel(((j >> tgt) & 1) != 0, ml.m11, ml.m00), xl[j]);
+            y[j] = csel(ok && (j & creg) == creg, u, xl[j]);
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    } else {
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            y[j] = csel((j & creg) == creg, cmul(f, xl[j]), xl[j]);
+            __builtin_amdgcn_sched_barrier(0);
+        });
     }
-    if constexpr (RB > 1) {
-        if (p == 1) reg_diag<R, A, 1, PHASE>(v, mm);
-    }
-    if constexpr (RB > 2) {
-        if (p == 2) reg_diag<R, A, 2, PHASE>(v, mm);
-    }
-    if constexpr (RB > 3) {
-        if (p == 3) reg_diag<R, A, 3, PHASE>(v, mm);
-    }
+    arm_marker<H, C>();
 }
 
-template <class R, int A, int F>
-__device__ __forceinline__ void reg_pair_at(Cx<R> (&v)[A], const Mat2<R> &mm, int p) {
-    constexpr int RB = Log2<A>::v;
-    switch (p) {
-        case 0:
-            if constexpr (RB > 0) {
-                reg_pair<R, A, 0, F>(v, mm, 0);
-                break;
-            }
-            [[fallthrough]];
-        case 1:
-            if constexpr (RB > 1) {
-                reg_pair<R, A, 1, F>(v, mm, 0);
-                break;
-            }
-            [[fallthrough]];
-        case 2:
-            if constexpr (RB > 2) {
-                reg_pair<R, A, 2, F>(v, mm, 0);
-                break;
-            }
-            [[fallthrough]];
-        case 3:
-            if constexpr (RB > 3) {
-                reg_pair<R, A, 3, F>(v, mm, 0);
-                break;
-            }
-            [[fallthrough]];
+// One op from x into y (H: 0 when x is the phase's first array, 1 for the second), one
+// switch: every arm ends at one join.
+template <class R, int A, int H>
+__device__ __forceinline__ void reg_op(const Cx<R> (&x)[A], Cx<R> (&y)[A], const Mat2<R> &mm, int arm, int creg,
+                                       bool ok, int tgt, const Cx<R> &f) {
+#define QG_REG_ARM(c)                                          \
+    case c:                                                    \
+        arm_body<R, A, H, c>(x, y, mm, creg, ok, tgt, f);      \
+        break;
+    switch (arm) {
+        QG_REG_ARM(0) QG_REG_ARM(1) QG_REG_ARM(2) QG_REG_ARM(3) QG_REG_ARM(4) QG_REG_ARM(5) QG_REG_ARM(6)
+        QG_REG_ARM(7) QG_REG_ARM(8) QG_REG_ARM(9) QG_REG_ARM(10) QG_REG_ARM(11) QG_REG_ARM(12) QG_REG_ARM(13)
+        QG_REG_ARM(14) QG_REG_ARM(15) QG_REG_ARM(16) QG_REG_ARM(17) QG_REG_ARM(18) QG_REG_ARM(19)
+        QG_REG_ARM(20) QG_REG_ARM(21) QG_REG_ARM(22) QG_REG_ARM(23) QG_REG_ARM(24) QG_REG_ARM(25)
+        QG_REG_ARM(26) QG_REG_ARM(27) QG_REG_ARM(28) QG_REG_ARM(29) QG_REG_ARM(30) QG_REG_ARM(31)
+        QG_REG_ARM(32) QG_REG_ARM(33)
         default: __builtin_unreachable();
     }
+#undef QG_REG_ARM
+}
+
+// The phase's amplitudes back to LDS, times the collected per-thread factor if there is one.
+template <class R, int A, int H>
+__device__ __forceinline__ void phase_store(Cx<R> *t, const Cx<R> (&v)[A], const PhaseBits &P, unsigned off,
+                                            const Cx<R> &tf, bool pending) {
+    constexpr int RB = Log2<A>::v;
+    if (pending) {
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            t[off | reg_bits<RB>(P, j)] = cmul(tf, v[j]);
+        });
+    } else {
+        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            t[off | reg_bits<RB>(P, j)] = v[j];
+        });
+    }
+    arm_marker<H, 63>();
 }
 
 __host__ __device__ constexpr bool is_pair_kind(int k) { return k == kOpPair || k == kOpPairSwap || k == kOpPairReal; }
+
+// One op of a register phase, decoded for reg_op.
+template <class R>
+struct RegStep {
+    Mat2<R> mm;
+    Cx<R> f;          // kArmFactorReg: the factor
+    int arm, creg, tgt;
+    bool ok;          // the thread passes the op's controls on lane and wave bits
+};
+
+// The first op of phase P at index k or later that changes amplitudes (P.n_ops if none),
+// decoded into d. Ops skipped on the way: those whose controls outside the tile fail (the
+// same for the whole workgroup), and per-thread factors on every register index, which are
+// multiplied into tf.
+template <class R, bool B>
+__device__ __forceinline__ int next_reg_op(const RegProgram<R> &prog, const RegPhase &P, int k, u64 base,
+                                           unsigned off, const Mat2<R> *tm, Cx<R> &tf, bool &pending,
+                                           RegStep<R> &d) {
+    const Cx<R> one = cx<R>(R(1), R(0));
+    for (; k < P.n_ops; ++k) {
+        const int oi = P.op_begin + k;
+        const u64 gctrl = prog.ops[oi].gctrl;
+        if ((base & gctrl) != gctrl) continue;
+        if constexpr (B)
+            d.mm = tm[2 * prog.ops[oi].gate];
+        else
+            d.mm = prog.ops[oi].m;
+        const unsigned cthr = prog.ops[oi].cthr;
+        const int creg = prog.ops[oi].creg, tgt = prog.ops[oi].target, kind = prog.ops[oi].kind;
+        // no controls on lane, wave or register bits: straight-line updates, no tests
+        const bool plain = cthr == 0 && creg == 0;
+        d.ok = (off & cthr) == cthr;
+        d.creg = creg;
+        d.tgt = tgt;
+        d.f = one;
+        if (kind <= kRegPairReal) {
+            d.arm = (plain ? 0 : kArmPairSel) + 4 * kind + tgt;
+        } else if (kind <= kRegDiagPhase) {
+            d.arm = plain ? (kind == kRegDiag ? kArmDiag : kArmDiagPhase) + tgt : kArmDiagCtrl;
+        } else {
+            const u64 src = kind == kRegDiagThread ? (u64)off : base;
+            Cx<R> f = csel(((src >> tgt) & 1) != 0, d.mm.m11, d.mm.m00);
+            f = csel(cthr == 0 || d.ok, f, one);
+            if (creg == 0) {
+                tf = cmul(tf, f);
+                pending = true;
+                continue;
+            }
+            d.f = f;
+            d.arm = kArmFactorReg;
+        }
+        return k;
+    }
+    return P.n_ops;
+}
 
 // Batched passes (B): the state vector holds many circuits of one gate structure, item i's
 // amplitudes at [i << item_shift, (i + 1) << item_shift), and no tile spans two items. Gate g
 // then takes its matrix from the tile item's table, imats[2 (item n_gates + g)] (tables hold U
 // and U^dagger per gate; imats points at the one the pass applies), instead of from the op.
+// One tile per workgroup: persistent workgroups that load the next tile into registers while
+// they work on the current one need 32 more VGPRs (3 workgroups per CU instead of 4) and
+// measured slower (30-qubit complex64 step 610 ms against 507 ms).
 template <class R, int M, int T, bool B>
 __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const RegProgram<R> prog, u64 first,
                                                 const Mat2<R> *__restrict__ imats, int n_gates, int item_shift) {
@@ -369,7 +418,8 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
     tile_to_lds<R, T, (1 << (M - lw)) / T>(t, a, base, prog.pos, M, prog.low);
     __syncthreads();
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    Cx<R> v[A];
+    const Cx<R> one = cx<R>(R(1), R(0));
+    Cx<R> va[A], vb[A];
     for (int ph = 0; ph < prog.n_phases; ++ph) {
         const RegPhase P = prog.phases[ph];
         unsigned off = 0;
@@ -379,80 +429,32 @@ __global__ __launch_bounds__(T) void k_tile_reg(Cx<R> *__restrict__ a, const Reg
         for (int k = 0; k < WB; ++k) off |= ((wave >> k) & 1u) << P.wb(k);
         static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            v[j] = t[off | reg_bits<RB>(P, j)];
+            va[j] = t[off | reg_bits<RB>(P, j)];
         });
         // Diagonal gates with no register bit among their bits multiply all of a thread's
         // amplitudes by one factor. They are collected in tf and applied once at the end of
         // the phase: tf commutes with every other gate of the phase, which mixes or scales
         // amplitudes of one thread only.
-        Cx<R> tf = cx<R>(R(1), R(0));
+        Cx<R> tf = one;
         bool pending = false;
-        for (int k = 0; k < P.n_ops; ++k) {
-            // plain locals: the lambdas below must not capture the kernel argument by address
-            const int oi = P.op_begin + k;
-            const u64 gctrl = prog.ops[oi].gctrl;
-            if ((base & gctrl) != gctrl) continue;   // same for the whole workgroup
-            Mat2<R> mm;
-            if constexpr (B)
-                mm = tm[2 * prog.ops[oi].gate];
-            else
-                mm = prog.ops[oi].m;
-            const unsigned cthr = prog.ops[oi].cthr;
-            const int creg = prog.ops[oi].creg, tgt = prog.ops[oi].target, kind = prog.ops[oi].kind;
-            // no controls on lane, wave or register bits: straight-line updates, no tests
-            const bool plain = cthr == 0 && creg == 0;
-            const bool ok = (off & cthr) == cthr;
-            if (kind <= kRegPairReal) {
-                // one-armed ifs here (not one switch): they cost register copies per gate but
-                // keep the kernel near 110 VGPRs, i.e. 4 workgroups per CU, which measured faster
-                if (plain) {
-                    if (kind == kRegPair) reg_pair_if<R, A, 0>(v, mm, tgt);
-                    if (kind == kRegPairSwap) reg_pair_if<R, A, 1>(v, mm, tgt);
-                    if (kind == kRegPairReal) reg_pair_if<R, A, 2>(v, mm, tgt);
-                } else {
-                    if (kind == kRegPair) reg_pair_sel_at<R, A, 0>(v, mm, creg, ok, tgt);
-                    if (kind == kRegPairSwap) reg_pair_sel_at<R, A, 1>(v, mm, creg, ok, tgt);
-                    if (kind == kRegPairReal) reg_pair_sel_at<R, A, 2>(v, mm, creg, ok, tgt);
-                }
-            } else if (kind <= kRegDiagPhase) {
-                // target on register position tgt: the factor of index j is a scalar choice
-                if (plain) {
-                    if (kind == kRegDiag) reg_diag_if<R, A, false>(v, mm, tgt);
-                    if (kind == kRegDiagPhase) reg_diag_if<R, A, true>(v, mm, tgt);
-                } else {
-                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                        constexpr int j = decltype(jc)::value;
-                        const Cx<R> y = cmul(csel(((j >> tgt) & 1) != 0, mm.m11, mm.m00), v[j]);
-                        v[j] = csel(ok && (j & creg) == creg, y, v[j]);
-                        __builtin_amdgcn_sched_barrier(0);
-                    });
-                }
-            } else {
-                const u64 src = kind == kRegDiagThread ? (u64)off : base;
-                Cx<R> f = csel(((src >> tgt) & 1) != 0, mm.m11, mm.m00);
-                f = csel(cthr == 0 || ok, f, cx<R>(R(1), R(0)));
-                if (creg == 0) {
-                    tf = cmul(tf, f);
-                    pending = true;
-                } else {
-                    static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                        constexpr int j = decltype(jc)::value;
-                        if ((j & creg) == creg) v[j] = cmul(f, v[j]);
-                        __builtin_amdgcn_sched_barrier(0);
-                    });
-                }
+        // ops in twos, va -> vb then vb -> va: only va is carried round the loop
+        int k = 0;
+        for (;;) {
+            RegStep<R> d;
+            k = next_reg_op<R, B>(prog, P, k, base, off, tm, tf, pending, d);
+            if (k >= P.n_ops) {
+                phase_store<R, A, 0>(t, va, P, off, tf, pending);
+                break;
             }
+            reg_op<R, A, 0>(va, vb, d.mm, d.arm, d.creg, d.ok, d.tgt, d.f);
+            k = next_reg_op<R, B>(prog, P, k + 1, base, off, tm, tf, pending, d);
+            if (k >= P.n_ops) {
+                phase_store<R, A, 1>(t, vb, P, off, tf, pending);
+                break;
+            }
+            reg_op<R, A, 1>(vb, va, d.mm, d.arm, d.creg, d.ok, d.tgt, d.f);
+            ++k;
         }
-        if (pending) {
-            static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-                constexpr int j = decltype(jc)::value;
-                v[j] = cmul(tf, v[j]);
-            });
-        }
-        static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
-            constexpr int j = decltype(jc)::value;
-            t[off | reg_bits<RB>(P, j)] = v[j];
-        });
         __syncthreads();
     }
     lds_to_tile<R, T, (1 << (M - lw)) / T>(a, t, base, prog.pos, M, prog.low);

@@ -162,20 +162,22 @@ def test_fusion_mode_checked(hip_module):
 
 
 @pytest.mark.parametrize('dtype', DTYPES)
-@pytest.mark.parametrize('registers', [True, False])
-def test_register_and_lds_tiles(hip_module, port_module, dtype, registers):
-    """Full-size tiles in registers (phases of up to 4 / 3 register-bit targets) and in LDS."""
+@pytest.mark.parametrize('registers,bits', [(True, 0), (True, 13), (False, 0)])
+def test_register_and_lds_tiles(hip_module, port_module, dtype, registers, bits):
+    """Full-size tiles in registers (phases of up to 4 / 3 register-bit targets), the wide
+    register tiles (13 / 12 bits, 512 threads; the default from 28 qubits, asked for here by
+    width), and tiles in LDS."""
     n = 15
     q, ops = _mixed_circuit(n, 500, seed=77)
     ref = _sim(port_module, ops, q, 'one_static', np.float64)
     hip_module.set_register_tiles(registers)
     try:
-        with _Fusion(hip_module, 'tile'):
+        with _Fusion(hip_module, 'tile', bits):
             got = _sim(hip_module, ops, q, 'one_static', dtype)
             err = max_abs(got.qubits.states[:], ref.qubits.states[:])
     finally:
         hip_module.set_register_tiles(True)
-    print('registers', registers, dtype.__name__, err)
+    print('registers', registers, bits, dtype.__name__, err)
     assert err < tol(dtype), err
 
 
