@@ -328,6 +328,9 @@ template <class R, int A, int H>
 __device__ __forceinline__ void phase_store(Cx<R> *t, const Cx<R> (&v)[A], const PhaseBits &P, unsigned off,
                                             const Cx<R> &tf, bool pending) {
     constexpr int RB = Log2<A>::v;
+    // the store addresses are formed here again (one OR each) from an opaque copy of off: the
+    // compiler would otherwise keep the phase load's A addresses live through every op
+    asm volatile("; phase store %1" : "+v"(off) : "n"(H));
     if (pending) {
         static_for<0, A>([&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
