@@ -87,9 +87,19 @@ __device__ __forceinline__ bool amp_on(int j, unsigned amp_ctrl) {
 }
 
 // Partner in the other half, `dist` units away.
-template <class R, int NI, int B>
+// SPREAD: even blocks take the first half of the launch's blocks and odd blocks the second
+// (a bijection for any grid size), so the blocks in flight read four runs of the state
+// instead of two. With partners 16 MiB or more apart the two runs of a plain launch cover
+// fewer HBM channels: at 30 qubits (complex64) targets 21..23 ran at 5.8-5.9 TB/s against
+// 6.5 TB/s for targets up to 20; spread, targets 21..28 gain 2-4% (targets up to 20 lose).
+template <class R, int NI, int B, bool SPREAD = false>
 __global__ __launch_bounds__(B) void k_strided_pair(Cx<R> *__restrict__ a, const GateArgs<R> g, u64 first) {
-    const u64 w = first + (u64)blockIdx.x * B + threadIdx.x;
+    u64 blk = blockIdx.x;
+    if constexpr (SPREAD) {
+        const u64 half = ((u64)gridDim.x + 1) / 2;
+        blk = (blk & 1) ? half + blk / 2 : blk / 2;
+    }
+    const u64 w = first + blk * B + threadIdx.x;
     if (w >= g.n_items) return;
     const u64 u0 = scatter<NI>(w, g.ins) | g.item_set, u1 = u0 | g.dist;
     Amps<R> x0 = load_unit(a, u0);
@@ -301,11 +311,14 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
     g.dist = u64(1) << tu;
     g.n_items = u64(1) << (unit_bits - (int)pos.size());
     const bool two_waves = tu == 6 || tu == 7;
+    const bool spread = (g.dist * sizeof(Cx<R>) * Unit<R>::W) >= (u64(16) << 20);   // partners >= 16 MiB apart
     with_ni((int)pos.size(), [&](auto ni) {
         constexpr int NI = decltype(ni)::value;
         for_each_piece(g.n_items, [&](u64 first, u64 count) {
             if (two_waves)
                 hipLaunchKernelGGL((k_strided_pair<R, NI, 128>), grid_for(count, 128), 128, 0, s, a, g, first);
+            else if (spread)
+                hipLaunchKernelGGL((k_strided_pair<R, NI, 64, true>), grid_for(count, 64), 64, 0, s, a, g, first);
             else
                 hipLaunchKernelGGL((k_strided_pair<R, NI, 64>), grid_for(count, 64), 64, 0, s, a, g, first);
         });
