@@ -171,11 +171,21 @@ constexpr int kChunk = 16;
 constexpr int kTileChunks = 256;
 constexpr int kTile = kChunk * kTileChunks;   // 4096 bins
 constexpr int kSuperTiles = 1024;             // 2^22 bins per super-tile
-// Threads per tile block: 8 bins each. With the tile's 37 KB of LDS four blocks fit on a CU,
-// so 512 threads give twice the wavefronts of 256 (the first 256 threads sum the chunks).
+// Threads per tile block: 8 bins each, 512 threads (the first 256 sum the chunks); the
+// persistent grid has two blocks per CU (pool_build).
 constexpr int kPoolThreads = 512;
 constexpr int kPoolRows = kTile / kPoolThreads;
 static_assert(kPoolThreads >= kTileChunks, "one thread per chunk for the chunk sums");
+
+// One DPP move of a double: lane l receives v of the lane that control CTRL names, in the
+// rows enabled by ROWS; lanes with no source lane and disabled rows receive 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 struct StateSrc {};
 struct ProbSrc {};
@@ -203,7 +213,7 @@ __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restr
     typedef typename PU::T T;
     __shared__ double sp[kTile + kTile / kChunk];   // one pad slot per chunk
     __shared__ double sc[kTileChunks];
-    __shared__ i64 sl[kPoolThreads / 64];
+    __shared__ int sl[kTileChunks];
     const T *src = FROM_STATE ? reinterpret_cast<const T *>(a) : reinterpret_cast<const T *>(p);
     const i64 n_units = n_bins / V;   // n_bins is a power of two >= 2
     // Loads are unconditional (units past the end read the last unit, and their values are
@@ -238,61 +248,66 @@ __global__ __launch_bounds__(kPoolThreads) void k_pool_tile(const Cx<R> *__restr
                 const int li = (r * kPoolThreads + threadIdx.x) * V + j;
                 const i64 bin = base + li;
                 // probabilities from the state, and zeros for the padding up to whole tiles
-                if (STORE_P && (FROM_STATE || bin >= n_bins)) p[bin] = v[j];
+                if (STORE_P && (FROM_STATE || bin >= n_bins)) __builtin_nontemporal_store(v[j], p + bin);
                 sp[li + li / kChunk] = v[j];
             }
         }
         __syncthreads();
-        // serial sum over this thread's chunk (threads below kTileChunks)
+        // serial sum over this thread's chunk (threads below kTileChunks) and the local index of
+        // its last bin with p > 0 (-1: none)
         const int c = threadIdx.x;
-        i64 last = -1;
         if (c < kTileChunks) {
             double s = 0.;
+            int last = -1;
             for (int k = 0; k < kChunk; ++k) {
                 const int li = c * kChunk + k;
                 const double x = sp[li + li / kChunk];
                 s = (k == 0) ? x : s + x;
-                if (x > 0.) last = base + li;
+                if (x > 0.) last = li;
             }
             sc[c] = s;
+            sl[c] = last;
         }
-        // the tile's last nonzero bin: maximum over the wavefront, then over the waves (below)
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) last = max(last, (i64)__shfl_xor((long long)last, d, 64));
-        if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = last;
         __syncthreads();
-        // inclusive scan of the chunk totals by one wavefront, in a fixed order: each lane scans
-        // four consecutive chunks, the lanes' totals are scanned across the wavefront
-        // (inclusive, doubling distances), and each lane adds the total of the lanes before it
+        // One wavefront scans the chunk totals in a fixed order: each lane adds up four
+        // consecutive chunks, the lanes' totals are scanned across the wavefront with DPP
+        // moves (within rows of 16 at distances 1, 2, 4, 8, then row 0's total into row 1, row
+        // 2's into row 3, rows 0-1's into rows 2-3), and each lane adds the total of the lanes
+        // before it (the scan shifted by one lane) to its four partial sums. The same wavefront
+        // stores the chunk prefixes, the tile total (its last chunk prefix, so the levels above
+        // agree with it exactly) and the tile's last bin with p > 0 (the highest lane holding
+        // one, by ballot). The next tile's chunk sums rewrite sc and sl only after the barrier
+        // that this wavefront reaches once it is done here.
         static_assert(kTileChunks == 4 * 64, "one wavefront scans the chunk totals, four per lane");
         if (threadIdx.x < 64) {
             const int l = threadIdx.x;
             const double q0 = sc[4 * l], q1 = q0 + sc[4 * l + 1], q2 = q1 + sc[4 * l + 2], q3 = q2 + sc[4 * l + 3];
             double incl = q3;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const double y = __shfl_up(incl, (unsigned)d, 64);
-                if (l >= d) incl += y;
-            }
-            double before = __shfl_up(incl, 1u, 64);
-            if (l == 0) before = 0.;
+            incl += dpp_f64<0x111, 0xf>(incl);   // row_shr:1
+            incl += dpp_f64<0x112, 0xf>(incl);   // row_shr:2
+            incl += dpp_f64<0x114, 0xf>(incl);   // row_shr:4
+            incl += dpp_f64<0x118, 0xf>(incl);   // row_shr:8
+            incl += dpp_f64<0x142, 0xa>(incl);   // row_bcast:15 into rows 1 and 3
+            incl += dpp_f64<0x143, 0xc>(incl);   // row_bcast:31 into rows 2 and 3
+            const double before = dpp_f64<0x138, 0xf>(incl);   // wave_shr:1, lane 0: 0
+            // the prefixes go back to sc (this wavefront has read it; its LDS accesses complete
+            // in order) and out 64 consecutive chunks a store, non-temporal: full 512-byte runs
+            // (four 8-byte stores a lane at a 32-byte stride, the tile scan took 1.79 ms at 30
+            // qubits, against 1.60 ms)
             sc[4 * l] = before + q0;
             sc[4 * l + 1] = before + q1;
             sc[4 * l + 2] = before + q2;
             sc[4 * l + 3] = before + q3;
-            // the tile total is its last chunk prefix, so the levels above agree with it exactly
-            if (l == 63) t_tot[tile] = before + q3;
-            if (l == 0) {
-                i64 m = sl[0];
+            double *eo = e_inc + tile * kTileChunks;
 #pragma unroll
-                for (int w = 1; w < kPoolThreads / 64; ++w) m = max(m, sl[w]);
-                t_last[tile] = m;
-            }
+            for (int i = 0; i < 4; ++i) __builtin_nontemporal_store(sc[64 * i + l], eo + 64 * i + l);
+            if (l == 63) t_tot[tile] = before + q3;
+            const int m = max(max(sl[4 * l], sl[4 * l + 1]), max(sl[4 * l + 2], sl[4 * l + 3]));
+            const unsigned long long has = __ballot(m >= 0);
+            const int hl = has ? 63 - __builtin_clzll(has) : 0;   // highest lane with a nonzero bin
+            const int ml = __builtin_amdgcn_readlane(m, hl);
+            if (l == 0) t_last[tile] = has ? base + ml : (i64)-1;
         }
-        __syncthreads();
-        if (c < kTileChunks) e_inc[tile * kTileChunks + c] = sc[c];
-        // sp and sc are rewritten for the next tile only after every thread passed this point
-        __syncthreads();
     };
     // two register buffers used in turn (no copies between them, whose reads would wait for
     // the loads in flight): the next tile's loads are issued before this tile's stores, and
@@ -479,8 +494,9 @@ template <class R, bool FROM_STATE, bool STORE_P>
 void pool_build(PoolData &pd, const Cx<R> *a) {
     hipStream_t s = stream();
     double *d_total = pd.h_inc + pd.n_supers + 1;
-    // four workgroups per compute unit fit the tile's 37 KB of LDS
-    const i64 grid = std::min<i64>(pd.n_tiles, (i64)runtime().n_cus * 4);
+    // two workgroups per compute unit (four fit the tile's 37 KB of LDS, and measured slower:
+    // a 30-qubit complex64 scan 1.60 ms with two, 1.71 ms with four)
+    const i64 grid = std::min<i64>(pd.n_tiles, (i64)runtime().n_cus * 2);
     hipLaunchKernelGGL((k_pool_tile<R, FROM_STATE, STORE_P>), (unsigned)grid, kPoolThreads, 0, s, a, pd.n,
                        pd.p, pd.e_inc, pd.m_inc, pd.t_last, pd.n_tiles);
     QG_HIP(hipGetLastError());

@@ -2,8 +2,10 @@
 
 k_pool_tile / k_pool_super / k_pool_top (qgate_amd/csrc/k_getter.hip) build a three-level CDF:
 chunks of 16 bins summed serially, the 256 chunk sums of a tile scanned by one wavefront (four
-chunks per lane, a doubling scan across lanes, each lane's prefix added to its four partial
-sums), tiles and super-tiles scanned serially; k_pool_sample searches it level by level. This
+chunks per lane; the lanes' totals scanned with DPP moves: within rows of 16 at distances 1, 2,
+4, 8, then row 0's total into row 1 and row 2's into row 3, then lane 31's into rows 2 and 3;
+each lane's exclusive prefix, the scan shifted by one lane, added to its four partial sums),
+tiles and super-tiles scanned serially; k_pool_sample searches it level by level. This
 emulation repeats those operations in the same order (numpy float64 is IEEE binary64, as the
 GPU's FP64), so it checks the invariants the kernels rely on, for zeros, sparse and dense
 inputs and pools smaller than one tile:
@@ -47,13 +49,18 @@ def build(p):
             q[l, 2] = q[l, 1] + sc[4 * l + 2]
             q[l, 3] = q[l, 2] + sc[4 * l + 3]
         incl = q[:, 3].copy()
-        d = 1
-        while d < 64:
-            prev = incl.copy()          # __shfl_up reads the values before this step
-            for l in range(d, 64):
-                incl[l] = incl[l] + prev[l - d]
-            d *= 2
-        before = np.concatenate([[0.], incl[:-1]])
+        for d in (1, 2, 4, 8):          # DPP row_shr:d, within rows of 16 lanes
+            prev = incl.copy()          # a DPP move reads the values before this step
+            for l in range(64):
+                if l % 16 >= d:
+                    incl[l] = incl[l] + prev[l - d]
+        prev = incl.copy()              # row_bcast:15 into rows 1 and 3
+        for l in list(range(16, 32)) + list(range(48, 64)):
+            incl[l] = incl[l] + prev[(l // 16) * 16 - 1]
+        prev = incl.copy()              # row_bcast:31 into rows 2 and 3
+        for l in range(32, 64):
+            incl[l] = incl[l] + prev[31]
+        before = np.concatenate([[0.], incl[:-1]])   # wave_shr:1
         for l in range(64):
             for j in range(4):
                 e_inc[t * TILE_CHUNKS + 4 * l + j] = before[l] + q[l, j]
