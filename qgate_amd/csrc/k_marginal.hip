@@ -18,6 +18,7 @@
 // return the same bits.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "qg_runtime.hpp"
@@ -69,6 +70,19 @@ struct MargArgs {
 };
 
 // One wavefront per 2^c items; item i = (kept-upper group, hidden-upper index), hidden fastest.
+// The sums of one unit's two kept amplitudes (k even): one 16-byte non-temporal store for FP64
+// output, so a wavefront's store covers 1 KiB contiguous (two 8-byte stores a lane each covered
+// half of every 16 bytes).
+template <class O>
+__device__ __forceinline__ void store_pair(O *out, u64 k, double s0, double s1) {
+    if constexpr (std::is_same<O, double>::value) {
+        __builtin_nontemporal_store((f64x2){s0, s1}, reinterpret_cast<f64x2 *>(out + k));
+    } else {
+        out[k] = (O)s0;
+        out[k + 1] = (O)s1;
+    }
+}
+
 // Unsliced: out[k] for each whole group. Sliced: partial[slice * n_out + k].
 template <class R, class O>
 __global__ __launch_bounds__(kMargThreads) void k_marginal(const Cx<R> *__restrict__ a, const MargArgs g,
@@ -120,9 +134,10 @@ __global__ __launch_bounds__(kMargThreads) void k_marginal(const Cx<R> *__restri
                     if (g.sliced) {
                         partial[slice * g.n_out + k] = s0;
                         if (W == 2 && !g.h_in_unit) partial[slice * g.n_out + k + 1] = s1;
+                    } else if (W == 2 && !g.h_in_unit) {
+                        store_pair(out, k, s0, s1);
                     } else {
                         out[k] = (O)s0;
-                        if (W == 2 && !g.h_in_unit) out[k + 1] = (O)s1;
                     }
                 }
                 ++grp;
@@ -188,8 +203,10 @@ __global__ __launch_bounds__(kMargThreads) void k_marginal_inner(const Cx<R> *__
             }
             if (writer) {
                 const u64 k = ((grp + i) << g.nk_low) | k_low;
-                out[k] = (O)t0;
-                if (W == 2 && !g.h_in_unit) out[k + 1] = (O)t1;
+                if (W == 2 && !g.h_in_unit)
+                    store_pair(out, k, t0, t1);
+                else
+                    out[k] = (O)t0;
             }
         }
         grp += G;
