@@ -438,8 +438,7 @@ bool use_dense_tile(int k, int n_lanes) {
 // Tile layout of a block on `lanes` (matrix index bit j <-> lanes[j]): the block's lanes and
 // the lowest other lanes up to M tile bits.
 template <class R>
-DenseTileArgs dense_tile_args(int n_lanes, const std::vector<int> &lanes) {
-    constexpr int M = DenseTileCfg<R>::M;
+DenseTileArgs dense_tile_args(int n_lanes, const std::vector<int> &lanes, int M = DenseTileCfg<R>::M) {
     std::vector<int> bits(lanes);
     for (int b = 0; (int)bits.size() < M && b < n_lanes; ++b)
         if (std::find(bits.begin(), bits.end(), b) == bits.end()) bits.push_back(b);
@@ -506,6 +505,7 @@ struct Mfma<double> {
 //     (FP64) of group l % 16; blocks t and t + M / 32 hold the real and imaginary part of the
 //     same amplitude.
 // Persistent workgroups stride over the tasks, so W is staged into LDS once per workgroup.
+// (complex128 5-qubit blocks on states of 10 qubits or more take k_unitary_tile_mfma.)
 template <class R, int K>
 __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<R> *__restrict__ a, const R *__restrict__ wcol,
                                                          const UnitaryArgs<R> g, u64 n_tasks) {
@@ -575,6 +575,118 @@ __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<R> *__restrict__ a, 
         run(t1, br, bi);
         task = t2;
     }
+}
+
+// Dense 5-qubit complex128 block on the FP64 matrix cores, over tiles: a tile of 2^10
+// amplitudes (16 KiB: the block's lanes and the lowest other lanes, loaded and stored as rows
+// of 16-byte units, as k_unitary_tile) goes to LDS, and each wavefront takes tasks of 16 of its
+// 32 groups: the B operands come from the tile, the steps are those of k_unitary_mfma, and the
+// results go back into the tile in place (a wavefront reads all of a task's amplitudes before
+// it writes them, and tasks share no group). Persistent workgroups stride over the tiles,
+// staging W (36 KiB) once. Measured on a 30-qubit state: 6.80 ms against 7.78 ms for the
+// per-group gathers of k_unitary_mfma, whose 16 runs of 256 B per task cap them below the
+// streaming rate; 32 KiB tiles took 7.8 ms, 8 KiB tiles 9.8 ms, and loading the next tile into
+// registers during the products 7.4-8.1 ms (its loads then mix with the stores). The tile's
+// loads are issued together in straight code: through the generic tile_to_lds / lds_to_tile
+// the compiler split them around the stores and waited inside the task loop (7.8 ms).
+// complex64 blocks gain nothing from tiles (3.6-3.7 ms against 3.6 ms on any tile size) and
+// keep the gathers.
+template <class R, int K, int M>
+__global__ __launch_bounds__(kBlock) void k_unitary_tile_mfma(Cx<R> *__restrict__ a, const R *__restrict__ wcol,
+                                                              const DenseTileArgs g, u64 n_tiles) {
+    typedef typename Mfma<R>::Acc Acc;
+    constexpr int D = 1 << K, MR = 2 * D, NT = MR / 16, MS = MR + Mfma<R>::Pad;
+    constexpr int G = 1 << (M - K), TASKS = G / 16, T = kBlock, NW = T / 64;
+    constexpr int W = Unit<R>::W, lw = W == 2 ? 1 : 0, U = (1 << (M - lw)) / T;
+    static_assert(G % 16 == 0, "a tile holds whole tasks of 16 groups");
+    __shared__ Cx<R> t[1 << M];
+    __shared__ R w[MR * MS];   // w[c * MS + r] = W[r][c]
+    for (int i = threadIdx.x; i < MR * MR; i += T) w[(i / MR) * MS + i % MR] = wcol[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, p = lane >> 4;
+    // member offsets are ORs of per-bit offsets (fmem[4 q + p] = fmem[4 q] | fmem[p]); an output
+    // row is member 16 t + v + 4 p (FP32) or 16 t + 4 v + p (FP64)
+    const unsigned lane_in = g.fmem[p];
+    const unsigned lane_out = Mfma<R>::RowStep4 ? g.fmem[p] : g.fmem[4 * p];
+    // unit c T + tid of a tile (local index (c T + tid) << lw) sits at state offset
+    // tile_base | deposit(tid << lw) | deposit(c T << lw); all U loads of a thread are issued
+    // before its first LDS write
+    const u64 dt = deposit((unsigned)threadIdx.x << lw, g.pos, M, g.low);
+    for (u64 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const u64 base = tile_base(tile, g.pos, M) | dt;
+        {
+            Amps<R> x[U];
+#pragma unroll
+            for (int c = 0; c < U; ++c)
+                x[c] = load_unit(a, (base | deposit((unsigned)(c * T) << lw, g.pos, M, g.low)) >> lw);
+#pragma unroll
+            for (int c = 0; c < U; ++c)
+#pragma unroll
+                for (int v = 0; v < W; ++v) t[((unsigned)(c * T + (int)threadIdx.x) << lw) + v] = x[c].v[v];
+        }
+        __syncthreads();   // (the first one also orders the staging of W)
+        for (int task = wave; task < TASKS; task += NW) {
+            // local index of group task * 16 + j: deposited at the tile bits outside the block
+            unsigned gb = 0, qq = (unsigned)(task * 16 + j);
+            for (unsigned m = g.rest; m != 0; m &= m - 1) {
+                if (qq & 1u) gb |= m & (~m + 1);
+                qq >>= 1;
+            }
+            R xr[D / 4], xi[D / 4];
+#pragma unroll
+            for (int q = 0; q < D / 4; ++q) {
+                const Cx<R> v = t[gb | lane_in | g.fmem[4 * q]];
+                xr[q] = v.re;
+                xi[q] = v.im;
+            }
+            Acc acc[NT];
+#pragma unroll
+            for (int tt = 0; tt < NT; ++tt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[tt][v] = R(0);
+#pragma unroll
+            for (int kk = 0; kk < MR / 4; ++kk) {
+                const R b = kk < D / 4 ? xr[kk] : xi[kk - D / 4];
+                const R *col = w + (4 * kk + p) * MS + j;
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) acc[tt] = Mfma<R>::step(col[16 * tt], b, acc[tt]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int tt = 0; tt < NT / 2; ++tt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    t[gb | lane_out | g.fmem[16 * tt + (Mfma<R>::RowStep4 ? 4 * v : v)]] =
+                        cx<R>(acc[tt][v], acc[tt + NT / 2][v]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < U; ++c) {
+            Amps<R> y;
+#pragma unroll
+            for (int v = 0; v < W; ++v) y.v[v] = t[((unsigned)(c * T + (int)threadIdx.x) << lw) + v];
+            store_unit(a, (base | deposit((unsigned)(c * T) << lw, g.pos, M, g.low)) >> lw, y);
+        }
+        __syncthreads();   // every thread has read t for its stores before the next tile lands
+    }
+}
+
+constexpr int kMfmaTileBits = 10;   // tile of the complex128 5-qubit tile kernel
+
+// 5-qubit complex128 blocks on states of at least one tile take the tile kernel.
+template <class R>
+bool use_mfma_tile(int k, int n_lanes) {
+    return std::is_same<R, double>::value && k == 5 && n_lanes >= kMfmaTileBits;
+}
+
+u64 mfma_tile_grid() {
+    static const int per_cu = [] {
+        int n = 0;
+        QG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_unitary_tile_mfma<double, 5, kMfmaTileBits>,
+                                                            kBlock, 0));
+        return std::max(1, n);
+    }();
+    return (u64)per_cu * (u64)std::max(1, runtime().n_cus);
 }
 
 // Workgroups per compute unit of the matrix-core kernel (LDS of W: complex64 20 KiB for K = 5,
@@ -649,6 +761,15 @@ void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, co
     const UnitaryArgs<R> g = unitary_args<R>(n_lanes, lanes);
     const int k = g.k;
     hipStream_t s = stream();
+    if constexpr (std::is_same<R, double>::value) {
+        if (use_mfma_tile<R>(k, n_lanes)) {
+            const DenseTileArgs tg = dense_tile_args<R>(n_lanes, lanes, kMfmaTileBits);
+            const u64 n_tiles = u64(1) << (n_lanes - kMfmaTileBits);
+            hipLaunchKernelGGL((k_unitary_tile_mfma<R, 5, kMfmaTileBits>), (unsigned)std::min(n_tiles, mfma_tile_grid()),
+                               kBlock, 0, s, a, static_cast<const R *>(dmat), tg, n_tiles);
+            return;
+        }
+    }
     if (use_mfma<R>(k, n_lanes)) {
         const u64 n_tasks = g.n_groups / 16;
         const int per_cu = mfma_groups_per_cu<R>(k);
