@@ -505,7 +505,8 @@ struct Mfma<double> {
 //     (FP64) of group l % 16; blocks t and t + M / 32 hold the real and imaginary part of the
 //     same amplitude.
 // Persistent workgroups stride over the tasks, so W is staged into LDS once per workgroup.
-// (complex128 5-qubit blocks on states of 10 qubits or more take k_unitary_tile_mfma.)
+// (complex128 blocks of 4 and 5 qubits on states of 11 / 10 qubits or more take
+// k_unitary_tile_mfma.)
 template <class R, int K>
 __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<R> *__restrict__ a, const R *__restrict__ wcol,
                                                          const UnitaryArgs<R> g, u64 n_tasks) {
@@ -577,20 +578,21 @@ __global__ __launch_bounds__(kBlock) void k_unitary_mfma(Cx<R> *__restrict__ a, 
     }
 }
 
-// Dense 5-qubit complex128 block on the FP64 matrix cores, over tiles: a tile of 2^10
-// amplitudes (16 KiB: the block's lanes and the lowest other lanes, loaded and stored as rows
-// of 16-byte units, as k_unitary_tile) goes to LDS, and each wavefront takes tasks of 16 of its
-// 32 groups: the B operands come from the tile, the steps are those of k_unitary_mfma, and the
-// results go back into the tile in place (a wavefront reads all of a task's amplitudes before
-// it writes them, and tasks share no group). Persistent workgroups stride over the tiles,
-// staging W (36 KiB) once. Measured on a 30-qubit state: 6.80 ms against 7.78 ms for the
-// per-group gathers of k_unitary_mfma, whose 16 runs of 256 B per task cap them below the
-// streaming rate; 32 KiB tiles took 7.8 ms, 8 KiB tiles 9.8 ms, and loading the next tile into
-// registers during the products 7.4-8.1 ms (its loads then mix with the stores). The tile's
-// loads are issued together in straight code: through the generic tile_to_lds / lds_to_tile
-// the compiler split them around the stores and waited inside the task loop (7.8 ms).
-// complex64 blocks gain nothing from tiles (3.6-3.7 ms against 3.6 ms on any tile size) and
-// keep the gathers.
+// Dense 4- or 5-qubit complex128 block on the FP64 matrix cores, over tiles: a tile of 2^M
+// amplitudes (the block's lanes and the lowest other lanes, loaded and stored as rows of
+// 16-byte units, as k_unitary_tile) goes to LDS, and each wavefront takes tasks of 16 of its
+// groups: the B operands come from the tile, the steps are those of k_unitary_mfma (two 16-row
+// output blocks for K = 4, four for K = 5), and the results go back into the tile in place (a
+// wavefront reads all of a task's amplitudes before it writes them, and tasks share no group).
+// Persistent workgroups stride over the tiles, staging W (10 / 36 KiB) once. Measured on a
+// 30-qubit state: K = 5 6.80 ms against 7.78 ms for the per-group gathers of k_unitary_mfma,
+// whose 16 runs of 256 B per task cap them below the streaming rate; K = 4 6.26 ms against
+// 6.82 ms for k_unitary_tile on the vector units (tile sizes and workgroup counts below).
+// Loading the next tile into registers during the products was slower (K = 5: 7.4-8.1 ms;
+// its loads then mix with the stores). The tile's loads are issued together in straight code:
+// through the generic tile_to_lds / lds_to_tile the compiler split them around the stores and
+// waited inside the task loop (7.8 ms). complex64 blocks gain nothing from tiles (K = 5:
+// 3.6-3.7 ms against 3.6 ms on any tile size) and keep the gathers.
 template <class R, int K, int M>
 __global__ __launch_bounds__(kBlock) void k_unitary_tile_mfma(Cx<R> *__restrict__ a, const R *__restrict__ wcol,
                                                               const DenseTileArgs g, u64 n_tiles) {
@@ -671,22 +673,43 @@ __global__ __launch_bounds__(kBlock) void k_unitary_tile_mfma(Cx<R> *__restrict_
     }
 }
 
-constexpr int kMfmaTileBits = 10;   // tile of the complex128 5-qubit tile kernel
+// Tile bits and resident workgroups per compute unit of k_unitary_tile_mfma<double, K>,
+// measured on the 30-qubit state (ms; tile KiB x workgroups per CU):
+//   K = 4: 32 x 2 6.26-6.38, 16 x 3..6 7.5-7.7, 32 x 3 7.19, 64 x 2 6.89, 32 x 1 8.66
+//          (k_unitary_tile on the vector units: 6.82);
+//   K = 5: 16 x 3 6.80-6.89, 32 x 2 6.98-7.07, 16 x 2 9.74, 64 x 1 9.5.
+// complex128 3-qubit blocks took 7.2 ms at best on the same kernel (one 16-row block holding
+// the real and imaginary rows) against 6.41 ms on the vector units, and stay there.
+template <int K>
+struct MfmaTileCfg;
+template <>
+struct MfmaTileCfg<4> {
+    static constexpr int M = 11, PerCU = 2;
+};
+template <>
+struct MfmaTileCfg<5> {
+    static constexpr int M = 10, PerCU = 3;
+};
 
-// 5-qubit complex128 blocks on states of at least one tile take the tile kernel.
+// 4- and 5-qubit complex128 blocks on states of at least one tile take the tile kernel.
 template <class R>
 bool use_mfma_tile(int k, int n_lanes) {
-    return std::is_same<R, double>::value && k == 5 && n_lanes >= kMfmaTileBits;
+    return std::is_same<R, double>::value && ((k == 4 && n_lanes >= MfmaTileCfg<4>::M) ||
+                                              (k == 5 && n_lanes >= MfmaTileCfg<5>::M));
 }
 
-u64 mfma_tile_grid() {
-    static const int per_cu = [] {
+template <int K>
+void launch_mfma_tile(Cx<double> *a, const double *w, int n_lanes, const std::vector<int> &lanes, hipStream_t s) {
+    constexpr int M = MfmaTileCfg<K>::M;
+    static const unsigned per_cu = [] {
         int n = 0;
-        QG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_unitary_tile_mfma<double, 5, kMfmaTileBits>,
-                                                            kBlock, 0));
-        return std::max(1, n);
+        QG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_unitary_tile_mfma<double, K, M>, kBlock, 0));
+        return (unsigned)std::max(1, std::min(n, MfmaTileCfg<K>::PerCU));
     }();
-    return (u64)per_cu * (u64)std::max(1, runtime().n_cus);
+    const DenseTileArgs tg = dense_tile_args<double>(n_lanes, lanes, M);
+    const u64 n_tiles = u64(1) << (n_lanes - M);
+    const u64 grid = std::min<u64>(n_tiles, (u64)per_cu * (u64)std::max(1, runtime().n_cus));
+    hipLaunchKernelGGL((k_unitary_tile_mfma<double, K, M>), (unsigned)grid, kBlock, 0, s, a, w, tg, n_tiles);
 }
 
 // Workgroups per compute unit of the matrix-core kernel (LDS of W: complex64 20 KiB for K = 5,
@@ -706,15 +729,20 @@ bool use_mfma(int k, int n_lanes) {
 // Device matrix of a block: complex row-major for the vector kernel (D x D Cx<R>), the real
 // form W column-major for the matrix-core kernel (M x M floats). Bytes of either.
 template <class R>
+bool real_form(int k, int n_lanes) {
+    return use_mfma<R>(k, n_lanes) || use_mfma_tile<R>(k, n_lanes);
+}
+
+template <class R>
 size_t unitary_bytes(int k, int n_lanes) {
     const size_t D = size_t(1) << k;
-    return use_mfma<R>(k, n_lanes) ? sizeof(R) * 4 * D * D : sizeof(Cx<R>) * D * D;
+    return real_form<R>(k, n_lanes) ? sizeof(R) * 4 * D * D : sizeof(Cx<R>) * D * D;
 }
 
 template <class R>
 void pack_unitary(int k, int n_lanes, const std::vector<std::complex<double>> &mat, char *out) {
     const int D = 1 << k;
-    if (use_mfma<R>(k, n_lanes)) {
+    if (real_form<R>(k, n_lanes)) {
         const int M = 2 * D;
         R *w = reinterpret_cast<R *>(out);
         for (int r = 0; r < M; ++r)
@@ -763,10 +791,11 @@ void launch_unitary_dev(Cx<R> *a, int n_lanes, const std::vector<int> &lanes, co
     hipStream_t s = stream();
     if constexpr (std::is_same<R, double>::value) {
         if (use_mfma_tile<R>(k, n_lanes)) {
-            const DenseTileArgs tg = dense_tile_args<R>(n_lanes, lanes, kMfmaTileBits);
-            const u64 n_tiles = u64(1) << (n_lanes - kMfmaTileBits);
-            hipLaunchKernelGGL((k_unitary_tile_mfma<R, 5, kMfmaTileBits>), (unsigned)std::min(n_tiles, mfma_tile_grid()),
-                               kBlock, 0, s, a, static_cast<const R *>(dmat), tg, n_tiles);
+            const R *w = static_cast<const R *>(dmat);
+            if (k == 5)
+                launch_mfma_tile<5>(a, w, n_lanes, lanes, s);
+            else
+                launch_mfma_tile<4>(a, w, n_lanes, lanes, s);
             return;
         }
     }

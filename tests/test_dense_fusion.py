@@ -82,9 +82,11 @@ def _apply_numpy(psi, n, lanes, u):
     (16, [15, 2]), (16, [13, 6, 0, 9]), (15, [14, 12, 1, 3, 7]), (16, [4, 5, 6]),
     # matrix cores over many tasks (complex128 on the FP64 matrix cores too)
     (16, [15, 2, 9, 4, 11, 0]), (17, [16, 3, 8, 12, 1]),
-    # complex128 5-lane blocks on 2^10-amplitude tiles: one tile, the block at the bottom of
-    # every tile, the block split between the lowest and the highest lanes
+    # complex128 4- and 5-lane blocks on matrix-core tiles of 2^11 / 2^10 amplitudes: one
+    # tile, the block at the bottom of every tile, the block split between the lowest and the
+    # highest lanes
     (10, [9, 0, 4, 6, 2]), (14, [0, 1, 2, 3, 4]), (14, [13, 12, 0, 1, 2]),
+    (11, [10, 1, 5, 3]), (14, [0, 1, 2, 3]), (13, [12, 0, 11, 1]),
 ])
 def test_apply_unitary_matches_numpy(hip_module, dtype, n, lanes):
     rs = np.random.RandomState(len(lanes) * 31 + n)
