@@ -205,6 +205,7 @@ def roofline(runner, n, amp, reps=4):
     mat = hipruntime.gate_matrix(runner.gt.U(0.3, 1.1, 2.3))
     per_target = []
     for t in range(n):
+        runner.proc.profile_gate(mat, runner.qs, [], t, 1)    # untimed warm-up launch
         per_target.append(runner.proc.profile_gate(mat, runner.qs, [], t, reps))
     avg_ms = float(np.mean(per_target))
     launch_bytes = 2 * amp * (1 << n)

@@ -19,8 +19,11 @@
 // Control bits above the unit are folded into the work-item index (zero bits inserted in
 // closed form, replacing the reference's uploaded permutation tables); control bits inside
 // a wavefront chunk or a unit are checked per lane or per amplitude.
-// Launch shape: one wavefront per block except for partners 64-128 units away (two); the
-// choice comes from per-target-bit bandwidth sweeps at 30 qubits.
+//   * partner 64 or 128 units away                         k_block_pair: a workgroup takes
+//     a contiguous chunk of twice the distance and exchanges partners through LDS.
+// Launch shapes (one wavefront per k_strided_pair block, two for partners 64-128 units apart;
+// the spread block order for partners 256-1024 units and 16 MiB or more apart) come from
+// per-target-bit bandwidth sweeps at 30 qubits.
 #include <algorithm>
 #include <complex>
 #include <cstring>
@@ -124,11 +127,56 @@ __global__ __launch_bounds__(64) void k_wave_pair(Cx<R> *__restrict__ a, const G
     const u64 w = first + blockIdx.x;   // one block per 64-unit chunk
     const int lane = threadIdx.x;
     const u64 unit = ((scatter<NI>(w, g.ins) | g.item_set) << 6) | (u64)lane;
-    const Amps<R> x = load_unit(a, unit);
+    // a lane whose control bits in the chunk are not all set neither loads nor stores: its
+    // partner (dist away on a bit that is no control) fails the same test
+    const bool act = ((u64)lane & g.lane_ctrl) == g.lane_ctrl;
+    Amps<R> x = {};
+    if (act) x = load_unit(a, unit);
     Amps<R> p;
 #pragma unroll
     for (int j = 0; j < Unit<R>::W; ++j) p.v[j] = shfl_xor_cx(x.v[j], (int)g.dist);
-    if (((u64)lane & g.lane_ctrl) != g.lane_ctrl) return;
+    if (!act) return;
+    const bool on = ((u64)lane & g.unit_chk) == g.unit_chk;   // off: written back unchanged
+    const bool hi = ((u64)lane & g.dist) != 0;
+    Amps<R> y = x;
+#pragma unroll
+    for (int j = 0; j < Unit<R>::W; ++j) {
+        if (!on || !amp_on<R>(j, g.amp_ctrl)) continue;
+        Cx<R> lo_v = hi ? p.v[j] : x.v[j];
+        Cx<R> hi_v = hi ? x.v[j] : p.v[j];
+        mix(g.m, lo_v, hi_v);
+        y.v[j] = hi ? hi_v : lo_v;
+    }
+    store_unit(a, unit, y);
+}
+
+// Partner 64 or 128 units away (unit bit tu = CB - 1): a workgroup of 2^CB threads takes a
+// contiguous chunk of 2^CB units, one per thread, and exchanges partners through LDS, so the
+// chunk is read and written as one run (as k_wave_pair's 1 KiB runs) instead of two runs of
+// 1 KiB per wavefront. 30-qubit complex64 targets 7 and 8: 6.80 TB/s against 6.15-6.19 for
+// k_strided_pair with two-wavefront blocks; for partners 256 units away or more, chunks of
+// 2 x dist units measured no faster than k_strided_pair (1024-unit chunks 6.37 TB/s against
+// 6.60 for the spread order) and chunks of several units per thread were slower.
+// Control bits of the unit index are split as in k_wave_pair (above the chunk: folded into
+// the chunk index; lane_ctrl: the lane sits out; unit_chk: tested, written back unchanged).
+template <class R, int NI, int CB>
+__global__ __launch_bounds__(1 << CB) void k_block_pair(Cx<R> *__restrict__ a, const GateArgs<R> g, u64 first) {
+    typedef typename UnitVec<R>::T V;
+    __shared__ V sx[1 << CB];
+    const u64 w = first + blockIdx.x;   // one block per chunk
+    const int lane = threadIdx.x;
+    const u64 unit = ((scatter<NI>(w, g.ins) | g.item_set) << CB) | (u64)lane;
+    const bool act = ((u64)lane & g.lane_ctrl) == g.lane_ctrl;   // partner fails alike
+    Amps<R> x = {};
+    if (act) x = load_unit(a, unit);
+    V v;
+    __builtin_memcpy(&v, &x, sizeof(V));
+    sx[lane] = v;
+    __syncthreads();
+    if (!act) return;
+    Amps<R> p;
+    v = sx[lane ^ (int)g.dist];
+    __builtin_memcpy(&p, &v, sizeof(V));
     const bool on = ((u64)lane & g.unit_chk) == g.unit_chk;   // off: written back unchanged
     const bool hi = ((u64)lane & g.dist) != 0;
     Amps<R> y = x;
@@ -303,6 +351,42 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
         return;
     }
 
+    // partner in a chunk of 2 dist units, unless a control inside the chunk lies on a whole
+    // line (unit bit 3 or more): folded into k_strided_pair's work-item index, such a control
+    // halves the lines moved, where the chunk would leave half its lanes idle (30-qubit
+    // complex64 CX, target 7, control 4..6: 5.9-6.0 TB/s strided against 4.9 in chunks)
+    bool chunk = (tu == 6 || tu == 7) && unit_bits > tu;
+    for (int p : unit_ctrl)
+        if (p >= kWholeLineBits && p <= tu) chunk = false;
+    if (chunk) {
+        const int cb = tu + 1;
+        std::vector<int> chunk_ctrl;
+        for (int p : unit_ctrl) {
+            if (p >= cb)
+                chunk_ctrl.push_back(p - cb);
+            else if (p < kWaveWholeLineBits)
+                g.unit_chk |= u64(1) << p;
+            else
+                g.lane_ctrl |= u64(1) << p;
+        }
+        g.ins = make_insert(chunk_ctrl);
+        for (int p : chunk_ctrl) g.item_set |= u64(1) << p;
+        g.dist = u64(1) << tu;
+        g.n_items = u64(1) << (unit_bits - cb - (int)chunk_ctrl.size());
+        with_ni((int)chunk_ctrl.size(), [&](auto ni) {
+            constexpr int NI = decltype(ni)::value;
+            // pieces in chunks of 2^cb work items (pieces are powers of two >= 2^12)
+            for_each_piece(g.n_items << cb, [&](u64 first, u64 count) {
+                if (cb == 7)
+                    hipLaunchKernelGGL((k_block_pair<R, NI, 7>), (unsigned)(count >> 7), 128, 0, s, a, g, first >> 7);
+                else
+                    hipLaunchKernelGGL((k_block_pair<R, NI, 8>), (unsigned)(count >> 8), 256, 0, s, a, g, first >> 8);
+            });
+        });
+        QG_HIP(hipGetLastError());
+        return;
+    }
+
     split_low(unit_ctrl);
     std::vector<int> pos(unit_ctrl);
     pos.push_back(tu);
@@ -310,8 +394,11 @@ void launch_gate(Cx<R> *a, int n_lanes, const GateMatrix &gm, const std::vector<
     for (int p : unit_ctrl) g.item_set |= u64(1) << p;
     g.dist = u64(1) << tu;
     g.n_items = u64(1) << (unit_bits - (int)pos.size());
+    // two wavefronts a block for partners 64-128 units apart (controlled gates that do not take
+    // k_block_pair); blocks in the spread order (the two halves of the grid interleaved) for
+    // partners 256 to 1024 units apart (target 9: 6.60 TB/s against 6.27) and 16 MiB or more
     const bool two_waves = tu == 6 || tu == 7;
-    const bool spread = (g.dist * sizeof(Cx<R>) * Unit<R>::W) >= (u64(16) << 20);   // partners >= 16 MiB apart
+    const bool spread = (tu >= 8 && tu <= 10) || (g.dist * sizeof(Cx<R>) * Unit<R>::W) >= (u64(16) << 20);
     with_ni((int)pos.size(), [&](auto ni) {
         constexpr int NI = decltype(ni)::value;
         for_each_piece(g.n_items, [&](u64 first, u64 count) {

@@ -90,6 +90,15 @@ def test_controls_on_low_bits(hip_module, port_module, dtype):
                     continue
                 ops.append(ctrl(q[c0], q[c1]).U3(*rs.uniform(0, 6.3, 3))(q[t]))
                 ops.append(ctrl(q[c0], q[c1]).S(q[t]))
+    # partners 64 / 128 units away (LDS chunk exchange): one control inside the chunk (tested
+    # per unit or per lane), one above it (folded into the chunk index)
+    for t in (6, 7, 8):
+        for c0 in (0, 1, 3, 5, 6):
+            for c1 in (9, 10):
+                if c0 == t:
+                    continue
+                ops.append(ctrl(q[c0], q[c1]).U3(*rs.uniform(0, 6.3, 3))(q[t]))
+                ops.append(ctrl(q[c0], q[c1]).X(q[t]))
     a = _sim(hip_module, ops, 'one_static', dtype, q)
     b = _sim(port_module, ops, 'one_static', dtype, q)
     assert max_abs(a.qubits.states[:], b.qubits.states[:]) < tol(dtype) * 10
