@@ -749,7 +749,9 @@ __device__ __forceinline__ void reg_sweep_pair_pos(Cx<R> (&vp)[A], Cx<R> (&vl)[A
 }
 
 // Pair op of form code 4 F + G (see sweep_form): fixed gates in the general, swap and real
-// forms, RY (real, Y) and RX (form 3, X); any other combination takes the general matrix.
+// forms, RY (real, Y) and RX (form 3, X); any other combination takes the general matrix with
+// the op's own generator G (an RX whose matrix happens to be real, RX(0) = 1, keeps its X
+// term).
 template <class R, int A>
 __device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A], const Mat2<R> &ud, int creg, bool ok,
                                                   bool plain, int p, int form, R &acc) {
@@ -759,10 +761,14 @@ __device__ __forceinline__ void reg_sweep_pair_at(Cx<R> (&vp)[A], Cx<R> (&vl)[A]
             case 4 * 3 + 1: reg_sweep_pair_pos<R, A, 3, 1, false>(vp, vl, ud, 0, true, p, acc); break;
             case 4 * 1 + 0: reg_sweep_pair_pos<R, A, 1, 0, false>(vp, vl, ud, 0, true, p, acc); break;
             case 4 * 2 + 0: reg_sweep_pair_pos<R, A, 2, 0, false>(vp, vl, ud, 0, true, p, acc); break;
-            case 1: reg_sweep_pair_pos<R, A, 0, 1, false>(vp, vl, ud, 0, true, p, acc); break;
-            case 2: reg_sweep_pair_pos<R, A, 0, 2, false>(vp, vl, ud, 0, true, p, acc); break;
-            case 3: reg_sweep_pair_pos<R, A, 0, 3, false>(vp, vl, ud, 0, true, p, acc); break;
-            default: reg_sweep_pair_pos<R, A, 0, 0, false>(vp, vl, ud, 0, true, p, acc); break;
+            default:
+                switch (form & 3) {
+                    case 1: reg_sweep_pair_pos<R, A, 0, 1, false>(vp, vl, ud, 0, true, p, acc); break;
+                    case 2: reg_sweep_pair_pos<R, A, 0, 2, false>(vp, vl, ud, 0, true, p, acc); break;
+                    case 3: reg_sweep_pair_pos<R, A, 0, 3, false>(vp, vl, ud, 0, true, p, acc); break;
+                    default: reg_sweep_pair_pos<R, A, 0, 0, false>(vp, vl, ud, 0, true, p, acc); break;
+                }
+                break;
         }
         return;
     }

@@ -316,3 +316,40 @@ def test_small_registers(hip_module, port_module, dtype, n):
     ref = ac.parameter_shift_gradient(port_module, spec, params, h)
     assert abs(value - ac.oracle_expectation(port_module, spec, params, h)) < VALUE_TOL[dtype] * 10
     assert np.max(np.abs(grad - ref)) < GRAD_TOL[dtype]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', DTYPES)
+@pytest.mark.parametrize('registers', [True, False])
+def test_rotations_at_special_angles(hip_module, port_module, dtype, registers):
+    """RX, RY, RZ at angles where the matrix takes a special form (0: the identity, a real
+    matrix for RX too; +-pi, pi / 2) and at random angles, on a 12-qubit register (register
+    sweep tiles of 2^11 / 2^10 amplitudes, and LDS tiles): every rotation keeps its
+    generator's gradient term whatever form its matrix has."""
+    n = 12
+    rs = np.random.RandomState(21)
+    spec = ac.CircuitSpec(n)
+    special = [0., math.pi, -math.pi, math.pi / 2]
+    params = []
+    for layer in range(2):
+        for q in range(n):
+            for rot in (spec.rx, spec.ry, spec.rz):
+                rot(q, len(params))
+                params.append(special[rs.randint(4)] if rs.randint(3) else rs.uniform(-math.pi, math.pi))
+        for q in range(n):
+            spec.cz(q, (q + 1) % n)
+    params = np.array(params)
+    params[:3] = 0.     # RX, RY, RZ of qubit 0 at exactly 0
+    terms = [(0.8, {q: 'X'}) for q in range(0, n, 2)] + [(-0.5, {q: 'Y'}) for q in range(1, n, 3)]
+    terms += [(0.6, {q: 'Z', q + 1: 'Z'}) for q in range(n - 1)]
+    h = PauliSum(terms)
+    hip_module.set_register_tiles(registers)
+    try:
+        value, grad = spec.build(dtype).set_fusion('tile').gradient(params, h)
+    finally:
+        hip_module.set_register_tiles(True)
+    ref = ac.parameter_shift_gradient(port_module, spec, params, h)
+    err = np.max(np.abs(grad - ref))
+    print(dtype.__name__, registers, 'grad err', err, 'at', int(np.argmax(np.abs(grad - ref))))
+    assert abs(value - ac.oracle_expectation(port_module, spec, params, h)) < VALUE_TOL[dtype] * 10
+    assert err < GRAD_TOL[dtype], err
