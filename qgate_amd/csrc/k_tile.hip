@@ -447,6 +447,8 @@ struct SweepCfg;
 // register tiles off (LdsBits), both holding a phi and a lambda tile. 12-bit complex64 tiles
 // (512 threads, 64 KiB of LDS) cut the 24-qubit ansatz's backward from 38 passes to 33 but
 // measured no faster: the sweep is bound by its per-gate instruction work, not by passes.
+// 128 threads on 11-bit tiles (16 amplitudes per state and thread, 203 VGPRs, two waves per
+// SIMD) made that backward slower, 7.4 -> 8.6 ms.
 template <>
 struct SweepCfg<float> {
     static constexpr int RegBits = 11, RegThreads = 256;   // 8 amplitudes per state and thread
