@@ -520,23 +520,60 @@ void apply_gate_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &g
     const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
     QG_REQUIRE(Unit<R>::W == 1 || low >= 1, "complex64 tiles need bit 0");
     const std::vector<QueuedGate> gates = merge_lane_runs(gates_in, n_lanes);
+    // Each pass scans the gates not yet applied in program order and takes every gate that
+    // commutes with all the gates it left out before it and whose target is a tile bit (or
+    // can become one, or the gate is diagonal). Two gates commute when on every lane they
+    // share both act diagonally there: a control, or the target of a diagonal gate. The
+    // gates a pass takes keep their program order; the ones it leaves keep theirs for the
+    // next passes. (Closing a pass at the first gate that does not fit, a 2000-gate random
+    // circuit on 30 qubits took 120 passes of about 14 ops.) The scan stops after kLookahead
+    // gates left out, or once every lane is held by one.
+    constexpr int kLookahead = 256;
+    const int n_gates = (int)gates.size();
+    std::vector<char> done(n_gates, 0);
+    std::vector<unsigned char> held(n_lanes);   // 0 free, 1 used diagonally, 2 otherwise
     PassPlan plan;
-    auto open = [&]() {
+    for (int first = 0; first < n_gates;) {
         plan.bits.clear();
         plan.gates.clear();
         for (int b = 0; b < low; ++b) plan.bits.push_back(b);
-    };
-    open();
-    for (const QueuedGate &g : gates) {
-        const bool need = !(is_diagonal(g.m) || plan.has(g.target));
-        if ((int)plan.bits.size() + (need ? 1 : 0) > mb || (int)plan.gates.size() >= C::MaxOps) {
-            launch_pass<R>(a, n_lanes, plan, low, mb);
-            open();
+        std::fill(held.begin(), held.end(), (unsigned char)0);
+        int left = 0, held_all = 0;
+        for (int i = first; i < n_gates && (int)plan.gates.size() < C::MaxOps; ++i) {
+            if (done[i]) continue;
+            const QueuedGate &g = gates[i];
+            const bool dg = is_diagonal(g.m);
+            bool conflict = dg ? held[g.target] == 2 : held[g.target] != 0;
+            for (int c : g.ctrl) conflict = conflict || held[c] == 2;
+            bool take = false;
+            if (!conflict) {
+                if (dg || plan.has(g.target)) {
+                    take = true;
+                } else if ((int)plan.bits.size() < mb) {
+                    plan.bits.push_back(g.target);
+                    take = true;
+                }
+            }
+            if (take) {
+                plan.gates.push_back(&g);
+                done[i] = 1;
+                continue;
+            }
+            auto hold = [&](int lane, unsigned char v) {
+                if (held[lane] < 2 && v == 2) ++held_all;
+                held[lane] = std::max(held[lane], v);
+            };
+            hold(g.target, dg ? 1 : 2);
+            for (int c : g.ctrl) hold(c, 1);
+            if (++left >= kLookahead || held_all == n_lanes) break;
         }
-        if (!is_diagonal(g.m) && !plan.has(g.target)) plan.bits.push_back(g.target);
-        plan.gates.push_back(&g);
+        if (plan.gates.empty()) {   // no tile bit left for the first gate (tile bits == low bits)
+            plan.gates.push_back(&gates[first]);   // a lone gate takes the streaming kernels
+            done[first] = 1;
+        }
+        launch_pass<R>(a, n_lanes, plan, low, mb);
+        while (first < n_gates && done[first]) ++first;
     }
-    if (!plan.gates.empty()) launch_pass<R>(a, n_lanes, plan, low, mb);
 }
 
 template void apply_gate_sequence<float>(Cx<float> *, int, const std::vector<QueuedGate> &, int, int);

@@ -21,7 +21,7 @@ from tests.helpers import max_abs, tol
 pytestmark = pytest.mark.gpu
 
 DTYPES = [np.float64, np.float32]
-SHAPES = [(0, -1), (4, 1), (6, 2), (8, 3), (10, 6)]
+SHAPES = [(0, -1), (4, 1), (6, 2), (8, 3), (10, 6), (5, 5)]   # (5, 5): no bit beyond the low ones
 
 
 def _sim(module, ops, q, prep, dtype, seed=5):
