@@ -253,8 +253,9 @@ def _strided_slice(module, r, n, dtype, count=1 << 20):
 
 def c128_step_timing(module, n, gates):
     """The same unfused step on a complex128 state (16 GiB at 30 qubits): one warm-up, one
-    timed run; algorithmic bytes with S = 16. Then the complex64 state of the same two runs
-    from |0...0> against it (complex64_vs_complex128)."""
+    timed run; algorithmic bytes with S = 16; then the step fused into tile passes (one
+    warm-up, one timed run). The complex64 state of the two unfused runs from |0...0> is
+    compared against the complex128 one (complex64_vs_complex128)."""
     r = Runner(module, n, np.float64)
     r.apply(gates)
     r.sync()
@@ -264,6 +265,17 @@ def c128_step_timing(module, n, gates):
     dt = time.perf_counter() - t0
     ref = _strided_slice(module, r, n, np.float64)
     ref_p0 = [r.proc.calc_probability(r.qs, q) for q in range(n)]
+    # the same step fused into tile passes: one warm-up, one timed run
+    module.set_fusion('tile')
+    r.proc.set_fusion('tile')
+    r.apply(gates)
+    r.sync()
+    t0 = time.perf_counter()
+    r.apply(gates)
+    r.sync()
+    fused_dt = time.perf_counter() - t0
+    module.set_fusion('off')
+    r.proc.set_fusion('off')
     r.delete()
     r32 = Runner(module, n, np.float32)
     r32.apply(gates)
@@ -279,7 +291,8 @@ def c128_step_timing(module, n, gates):
            'max_abs_amplitude': float(np.max(np.abs(ref)))}
     return {'ms_per_step': round(dt * 1e3, 1), 'GBs': round(step_bytes / dt / 1e9, 1),
             'frac_of_hbm_peak': round(step_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
-            'gates_per_s': round(len(gates) / dt, 1)}, err
+            'gates_per_s': round(len(gates) / dt, 1),
+            'fused_tile_ms_per_step': round(fused_dt * 1e3, 1)}, err
 
 
 def dense_timing(r, gates, step_s, step_bytes, n, widths=(2, 3, 4, 5, 6)):
