@@ -11,6 +11,8 @@ struct GateMatrix {
     std::complex<double> m[2][2];
 };
 
+inline bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
+
 // Factory id for a gate type name ("U", "U2", "U1", "ID", "X", "Y", "Z", "H", "S", "T",
 // "RX", "RY", "RZ", "ExpiI", "ExpiZ", "SH"), or -1.
 int matrix_factory_id(const char *name);

@@ -984,32 +984,65 @@ void launch_unitary(Cx<R> *a, int n_lanes, const std::vector<int> &lanes,
 template <class R>
 void apply_dense_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &gates, int width) {
     QG_REQUIRE(2 <= width && width <= kMaxFuse, "dense block width %d out of range", width);
-    // plan: greedy blocks of consecutive gates on at most `width` lanes
+    // plan: each block scans the gates not yet applied in program order and takes every gate
+    // that commutes with all the gates it left out before it (on every lane they share, both
+    // act diagonally: a control, or the target of a diagonal gate) and whose lanes, controls
+    // included, keep the block within `width` lanes. The gates of a block keep their program
+    // order, and so do the ones left for later blocks. (Blocks of consecutive gates only: the
+    // 2000-gate 30-qubit random circuit took 669 blocks at width 5, 541 at width 6; the scan
+    // makes them about 360 and 290.) A gate wider than a block goes alone. The scan stops
+    // after kLookahead gates left out or once every lane is held.
     struct Block {
         std::vector<int> lanes;
         std::vector<const QueuedGate *> gates;
     };
+    constexpr int kLookahead = 256;
+    const int n_gates = (int)gates.size();
+    std::vector<char> done(n_gates, 0);
+    std::vector<unsigned char> held(n_lanes);   // 0 free, 1 used diagonally, 2 otherwise
     std::vector<Block> blocks;
-    Block cur;
-    for (const QueuedGate &g : gates) {
-        std::vector<int> need(g.ctrl);
-        need.push_back(g.target);
-        std::vector<int> merged(cur.lanes);
-        for (int l : need)
-            if (std::find(merged.begin(), merged.end(), l) == merged.end()) merged.push_back(l);
-        if ((int)merged.size() > width && !cur.gates.empty()) {
-            blocks.push_back(cur);
-            cur = Block();
-            merged = need;
+    for (int first = 0; first < n_gates;) {
+        Block cur;
+        std::fill(held.begin(), held.end(), (unsigned char)0);
+        int left = 0, held_all = 0;
+        for (int i = first; i < n_gates; ++i) {
+            if (done[i]) continue;
+            const QueuedGate &g = gates[i];
+            const bool dg = is_diagonal(g.m);
+            bool conflict = dg ? held[g.target] == 2 : held[g.target] != 0;
+            for (int c : g.ctrl) conflict = conflict || held[c] == 2;
+            if (!conflict) {
+                std::vector<int> merged(cur.lanes);
+                auto add = [&](int l) {
+                    if (std::find(merged.begin(), merged.end(), l) == merged.end()) merged.push_back(l);
+                };
+                for (int c : g.ctrl) add(c);
+                add(g.target);
+                if ((int)merged.size() <= width) {
+                    cur.lanes.swap(merged);
+                    cur.gates.push_back(&g);
+                    done[i] = 1;
+                    continue;
+                }
+            }
+            auto hold = [&](int lane, unsigned char v) {
+                if (held[lane] < 2 && v == 2) ++held_all;
+                held[lane] = std::max(held[lane], v);
+            };
+            hold(g.target, dg ? 1 : 2);
+            for (int c : g.ctrl) hold(c, 1);
+            if (++left >= kLookahead || held_all == n_lanes) break;
         }
-        cur.lanes = merged;
-        cur.gates.push_back(&g);
-        if ((int)need.size() > width) {   // wider than a block: applied alone
-            blocks.push_back(cur);
-            cur = Block();
+        if (cur.gates.empty()) {   // the first gate is wider than a block: alone
+            const QueuedGate &g = gates[first];
+            cur.lanes = g.ctrl;
+            cur.lanes.push_back(g.target);
+            cur.gates.push_back(&g);
+            done[first] = 1;
         }
+        blocks.push_back(cur);
+        while (first < n_gates && done[first]) ++first;
     }
-    if (!cur.gates.empty()) blocks.push_back(cur);
     // matrices of the multi-gate blocks, uploaded together
     std::vector<size_t> offset(blocks.size(), 0);
     size_t total = 0;

@@ -93,8 +93,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_pass(Cx<R> *__restrict__ 
     store_tile_lds<R, kTileThreads, MB>(a, t, base, prog.pos, m, prog.low);
 }
 
-bool is_diagonal(const GateMatrix &g) { return g.m[0][1] == 0. && g.m[1][0] == 0.; }
-
 // Pauli X (1), Y (2), Z (3); zero matrix otherwise.
 void set_pauli(GateMatrix &g, int pauli) {
     const std::complex<double> J(0., 1.);
