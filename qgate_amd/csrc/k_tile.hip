@@ -585,34 +585,63 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
     const int cap = regs ? C::RegBits : C::LdsBits;
     const int mb = std::min(max_bits > 0 ? std::min(max_bits, cap) : cap, n_lanes);
     const int low = std::min(low_bits >= 0 ? low_bits : C::Low, mb);
-    // plan the passes: like apply_gate_sequence, with at most kMaxSweepSlots rotations a pass
+    // plan the passes as apply_gate_sequence does (each pass takes every gate not yet swept
+    // that commutes with the gates it left out before it and fits the tile), with at most
+    // kMaxSweepSlots rotations a pass. A gate counts as diagonal only if it is for every
+    // parameter value (a fixed gate, or generator Z), so a reordered rotation has the same
+    // derivative. `order` lists the gates pass by pass; a pass is a range [begin, end) of it.
     struct Plan {
         std::vector<int> bits;
         int begin, end, slots;
     };
+    constexpr int kLookahead = 256;
+    const int n_gates = (int)gates.size();
+    std::vector<int> order;
+    order.reserve(n_gates);
+    std::vector<char> done(n_gates, 0);
+    std::vector<unsigned char> held(n_lanes);   // 0 free, 1 used diagonally, 2 otherwise
     std::vector<Plan> plans;
-    Plan cur;
-    auto open = [&](int at) {
-        cur.bits.clear();
+    for (int first = 0; first < n_gates;) {
+        Plan cur;
         for (int b = 0; b < low; ++b) cur.bits.push_back(b);
-        cur.begin = cur.end = at;
+        cur.begin = (int)order.size();
         cur.slots = 0;
-    };
-    auto has = [&](int b) { return std::find(cur.bits.begin(), cur.bits.end(), b) != cur.bits.end(); };
-    open(0);
-    for (int i = 0; i < (int)gates.size(); ++i) {
-        const SweepGate &g = gates[i];
-        const bool diag = is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3);
-        const bool need = !(diag || has(g.target));
-        if ((int)cur.bits.size() + (need ? 1 : 0) > mb || (g.pauli != 0 && cur.slots == kMaxSweepSlots)) {
-            plans.push_back(cur);
-            open(i);
+        auto has = [&](int b) { return std::find(cur.bits.begin(), cur.bits.end(), b) != cur.bits.end(); };
+        std::fill(held.begin(), held.end(), (unsigned char)0);
+        int left = 0, held_all = 0;
+        for (int i = first; i < n_gates; ++i) {
+            if (done[i]) continue;
+            const SweepGate &g = gates[i];
+            const bool diag = is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3);
+            bool conflict = diag ? held[g.target] == 2 : held[g.target] != 0;
+            for (int c : g.ctrl) conflict = conflict || held[c] == 2;
+            const bool slot_ok = g.pauli == 0 || cur.slots < kMaxSweepSlots;
+            if (!conflict && slot_ok && (diag || has(g.target) || (int)cur.bits.size() < mb)) {
+                if (!(diag || has(g.target))) cur.bits.push_back(g.target);
+                if (g.pauli != 0) ++cur.slots;
+                order.push_back(i);
+                done[i] = 1;
+                continue;
+            }
+            auto hold = [&](int lane, unsigned char v) {
+                if (held[lane] < 2 && v == 2) ++held_all;
+                held[lane] = std::max(held[lane], v);
+            };
+            hold(g.target, diag ? 1 : 2);
+            for (int c : g.ctrl) hold(c, 1);
+            if (++left >= kLookahead || held_all == n_lanes) break;
         }
-        if (need) cur.bits.push_back(g.target);
-        if (g.pauli != 0) ++cur.slots;
-        cur.end = i + 1;
+        if ((int)order.size() == cur.begin) {   // no tile bit left (tile bits == low bits): as before,
+            const SweepGate &g = gates[first];     // the first gate widens the tile by its target
+            cur.bits.push_back(g.target);
+            if (g.pauli != 0) ++cur.slots;
+            order.push_back(first);
+            done[first] = 1;
+        }
+        cur.end = (int)order.size();
+        plans.push_back(cur);
+        while (first < n_gates && done[first]) ++first;
     }
-    if (cur.end > cur.begin) plans.push_back(cur);
 
     constexpr int A = (1 << C::RegBits) / C::RegThreads;
     // device tables: ops (pass by pass), phases of register passes, slot -> gate index
@@ -638,9 +667,82 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         int local[64];
         for (int b = 0; b < 64; ++b) local[b] = -1;
         for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
+        if (regs && ps.m == C::RegBits) {
+            // A register pass keeps its pair updates in order, and puts each diagonal gate at a
+            // position, between the pair updates on its lanes before and after it, whose phase
+            // has none of its tile lanes as a register bit: there it is a per-thread factor, not
+            // an update of both register arrays. (A pass that takes the gates of several
+            // layers otherwise meets a layer's RZ and CZ right after the RY that made their
+            // lanes register bits: the 24-qubit ansatz's complex64 sweep did 253 such updates
+            // instead of 58 and ran 5% slower in half the passes.) Diagonal gates commute with
+            // each other and with every pair update whose target is none of their lanes, as
+            // do their generators, so the states and the gradient terms are unchanged.
+            auto is_diag = [&](const SweepGate &g) { return is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3); };
+            std::vector<int> pairs, pair_local;
+            for (int i = pl.begin; i < pl.end; ++i)
+                if (!is_diag(gates[order[i]])) {
+                    pairs.push_back(order[i]);
+                    pair_local.push_back(local[gates[order[i]].target]);
+                }
+            const int np = (int)pairs.size();
+            if (np > 0) {
+                const std::vector<PhaseLayout> L =
+                    plan_phase_layouts(pair_local, ps.m, Log2<A>::v, Log2<C::RegThreads / 64>::v);
+                std::vector<int> phase_of(np, 0);
+                for (int ph = 0; ph < (int)L.size(); ++ph)
+                    for (int k = L[ph].begin; k < L[ph].end; ++k) phase_of[k] = ph;
+                // a gate after pair update gp - 1 (before the first: gp = 0) runs in its phase
+                auto gap_phase = [&](int gp) { return gp >= 1 ? phase_of[gp - 1] : 0; };
+                std::vector<std::vector<int>> at(np + 1);
+                int seen = 0;
+                for (int i = pl.begin; i < pl.end; ++i) {
+                    const int gi = order[i];
+                    const SweepGate &g = gates[gi];
+                    if (!is_diag(g)) {
+                        ++seen;
+                        continue;
+                    }
+                    std::vector<int> lanes(g.ctrl);
+                    lanes.push_back(g.target);
+                    auto hits = [&](int k) {
+                        return std::find(lanes.begin(), lanes.end(), gates[pairs[k]].target) != lanes.end();
+                    };
+                    int lo = 0, hi = np;
+                    for (int k = seen - 1; k >= 0; --k)
+                        if (hits(k)) {
+                            lo = k + 1;
+                            break;
+                        }
+                    for (int k = seen; k < np; ++k)
+                        if (hits(k)) {
+                            hi = k;
+                            break;
+                        }
+                    auto bad = [&](int gp) {
+                        const std::vector<int> &rg = L[gap_phase(gp)].regs;
+                        for (int l : lanes)
+                            if (local[l] >= 0 && std::find(rg.begin(), rg.end(), local[l]) != rg.end()) return true;
+                        return false;
+                    };
+                    int best = seen;
+                    for (int d = 1; bad(best) && (seen - d >= lo || seen + d <= hi); ++d) {
+                        if (seen - d >= lo && !bad(seen - d))
+                            best = seen - d;
+                        else if (seen + d <= hi && !bad(seen + d))
+                            best = seen + d;
+                    }
+                    at[best].push_back(gi);
+                }
+                int w = pl.begin;
+                for (int gp = 0; gp <= np; ++gp) {
+                    for (int gi : at[gp]) order[w++] = gi;
+                    if (gp < np) order[w++] = pairs[gp];
+                }
+            }
+        }
         int slots = 0;
         for (int i = pl.begin; i < pl.end; ++i) {
-            const SweepGate &g = gates[i];
+            const SweepGate &g = gates[order[i]];
             SweepOp<R> &op = hops[i];
             op.udag = round_matrix<R>(g.udag);
             GateMatrix gen;
@@ -665,7 +767,7 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
             }
             op.form = sweep_form(g.udag, g.pauli, op.kind != kOpPair);
             op.slot = g.pauli != 0 ? slots++ : -1;
-            if (g.pauli != 0) slot_gate.push_back(i);
+            if (g.pauli != 0) slot_gate.push_back(order[i]);
         }
         ps.n_slots = slots;
         const bool use_regs = regs && ps.m == C::RegBits;
