@@ -368,6 +368,74 @@ struct Layout {
     int begin, end;
 };
 
+// Passes over the gates of a circuit in a planned order: each pass takes every gate not yet
+// planned that commutes with the gates it left out before it (on every lane they share both
+// act diagonally: a control, or the target of a diagonal gate) and whose non-diagonal target
+// fits the tile, up to max_ops gates and max_counted counted ones. full[i] holds gate i's
+// lanes used non-diagonally (its target unless diagonal: the bits it needs in the tile),
+// diag[i] the others. `order` receives the gates pass by pass; the layouts' [begin, end)
+// index it. (Passes of consecutive gates only, as plan_layouts: the 4096-circuit batch of
+// 8..20-qubit ansatz circuits took about twice as many passes.) The scan stops after
+// kLookahead gates left out or once every lane is held.
+std::vector<Layout> plan_layouts_ahead(const std::vector<u64> &full, const std::vector<u64> &diag, int n, int mb,
+                                       int low, bool c64, int max_ops, const std::vector<char> *counted,
+                                       int max_counted, std::vector<int> &order) {
+    constexpr int kLookahead = 256;
+    const int ne = (int)full.size();
+    const u64 all = n >= 64 ? ~u64(0) : (u64(1) << n) - 1;
+    order.clear();
+    std::vector<char> done(ne, 0);
+    std::vector<Layout> out;
+    for (int first = 0; first < ne;) {
+        Layout cur;
+        cur.begin = (int)order.size();
+        u64 bits = (u64(1) << low) - 1, h1 = 0, h2 = 0;   // h2: lanes held non-diagonally
+        int n_counted = 0, left = 0;
+        for (int i = first; i < ne && (int)order.size() - cur.begin < max_ops; ++i) {
+            if (done[i]) continue;
+            QG_REQUIRE(__builtin_popcountll(full[i] | (c64 ? 1 : 0)) <= mb, "gate does not fit a tile");
+            const int c = counted && (*counted)[i] ? 1 : 0;
+            const bool conflict = (full[i] & (h1 | h2)) != 0 || (diag[i] & h2) != 0;
+            if (!conflict && n_counted + c <= max_counted && __builtin_popcountll(bits | full[i]) <= mb) {
+                bits |= full[i];
+                n_counted += c;
+                order.push_back(i);
+                done[i] = 1;
+                continue;
+            }
+            h2 |= full[i];
+            h1 |= diag[i];
+            if (++left >= kLookahead || (h2 & all) == all) break;
+        }
+        if ((int)order.size() == cur.begin) {   // the first gate does not fit beside the low bits
+            bits = full[first];
+            order.push_back(first);
+            done[first] = 1;
+        }
+        cur.end = (int)order.size();
+        for (int b = 0; b < n; ++b)
+            if ((bits >> b) & 1) cur.bits.push_back(b);
+        out.push_back(cur);
+        while (first < ne && done[first]) ++first;
+    }
+    if (out.empty()) {
+        Layout e;
+        e.begin = e.end = 0;
+        for (int b = 0; b < low; ++b) e.bits.push_back(b);
+        out.push_back(e);
+    }
+    for (Layout &l : out) {
+        if (c64 && std::find(l.bits.begin(), l.bits.end(), 0) == l.bits.end()) l.bits.push_back(0);
+        for (int b = 0; (int)l.bits.size() < mb && b < n; ++b)
+            if (std::find(l.bits.begin(), l.bits.end(), b) == l.bits.end()) l.bits.push_back(b);
+        std::sort(l.bits.begin(), l.bits.end());
+        int run = 0;
+        while (run < (int)l.bits.size() && l.bits[run] == run) ++run;
+        l.low = run;
+    }
+    return out;
+}
+
 // Passes over consecutive entries: need_masks[i] holds the bits entry i requires in the tile.
 // A pass also ends before it would hold more than max_ops entries, or more than max_counted
 // entries with counted[i] set (rotations of a sweep pass: one gradient slot each).
@@ -592,13 +660,23 @@ void build_group(Group &g, const Tuning &t) {
     const bool sregs = t.registers != 0 && ms == C::SweepBits && use_register_tiles();
 
     // forward passes
-    std::vector<u64> need(n_gates);
+    // (a rotation counts as diagonal only with generator Z: diagonal for every item's angle)
+    std::vector<u64> need(n_gates), dlanes(n_gates);
     for (int i = 0; i < n_gates; ++i) {
         const CircuitOp &op = g.ops[i];
         const bool diag = op.kind == 1 ? op.pauli == 3 : diag_matrix(op.mat);
         need[i] = diag ? 0 : (u64(1) << op.target);
+        dlanes[i] = diag ? (u64(1) << op.target) : 0;
+        for (int c : op.ctrl) dlanes[i] |= u64(1) << c;
     }
-    std::vector<Layout> fl = plan_layouts(need, n, mf, lowf, c64, fregs ? TC::MaxOps : 1 << 30);
+    auto lanes_of = [&](int gi) {
+        std::vector<int> l(g.ops[gi].ctrl);
+        l.push_back(g.ops[gi].target);
+        return l;
+    };
+    std::vector<int> ford;   // forward gates pass by pass
+    std::vector<Layout> fl =
+        plan_layouts_ahead(need, dlanes, n, mf, lowf, c64, fregs ? TC::MaxOps : 1 << 30, nullptr, 1 << 30, ford);
     std::vector<BOp> fops(std::max(1, n_gates));
     g.fpasses.clear();
     std::vector<RegProgram<R>> &progs = reg_programs<R>(g);
@@ -606,8 +684,14 @@ void build_group(Group &g, const Tuning &t) {
     for (const Layout &l : fl) {
         int local[64];
         BPass p = make_pass(l, local);
+        if (fregs) {   // diagonal gates where none of their lanes is a register bit
+            constexpr int A = (1 << TC::MaxBits) / kBatchRegThreads;
+            place_diagonals(
+                ford, l.begin, l.end, local, p.m, Log2<A>::v, Log2<kBatchRegThreads / 64>::v,
+                [&](int gi) { return need[gi] == 0; }, [&](int gi) { return g.ops[gi].target; }, lanes_of);
+        }
         for (int i = l.begin; i < l.end; ++i) {
-            const CircuitOp &op = g.ops[i];
+            const CircuitOp &op = g.ops[ford[i]];
             BOp &o = fops[i];
             o.gctrl = 0;
             o.lctrl = 0;
@@ -617,10 +701,10 @@ void build_group(Group &g, const Tuning &t) {
                 else
                     o.gctrl |= u64(1) << c;
             }
-            const bool diag = need[i] == 0;
+            const bool diag = need[ford[i]] == 0;
             o.kind = (short)(local[op.target] >= 0 ? (diag ? kOpDiagLocal : kOpPair) : kOpDiagOuter);
             o.target = (short)(local[op.target] >= 0 ? local[op.target] : op.target);
-            o.gate = i;
+            o.gate = ford[i];
             o.pauli = op.kind == 1 ? op.pauli : 0;
             o.slot = -1;
         }
@@ -638,21 +722,23 @@ void build_group(Group &g, const Tuning &t) {
                 in[k].gctrl = o.gctrl;
                 in[k].lctrl = o.lctrl;
                 in[k].target = o.target;
-                in[k].kind = reg_forward_kind<R>(g.ops[l.begin + k], o.kind);
+                in[k].kind = reg_forward_kind<R>(g.ops[ford[l.begin + k]], o.kind);
             }
             constexpr int A = (1 << TC::MaxBits) / kBatchRegThreads;
             QG_REQUIRE(plan_phases<R>(rp, in.data(), p.m, Log2<A>::v, Log2<kBatchRegThreads / 64>::v),
                        "batch: register pass with too many phases");
-            for (int k = 0; k < p.n_ops; ++k) rp.ops[k].gate = l.begin + k;
+            for (int k = 0; k < p.n_ops; ++k) rp.ops[k].gate = ford[l.begin + k];
             progs.push_back(rp);
         }
     }
     // sweep passes: the gates in reverse order
-    std::vector<u64> sneed(n_gates);
+    std::vector<u64> sneed(n_gates), sdl(n_gates);
     for (int i = 0; i < n_gates; ++i) sneed[i] = need[n_gates - 1 - i];
+    for (int i = 0; i < n_gates; ++i) sdl[i] = dlanes[n_gates - 1 - i];
     std::vector<char> srot(n_gates);
     for (int i = 0; i < n_gates; ++i) srot[i] = g.ops[n_gates - 1 - i].kind == 1 ? 1 : 0;
-    std::vector<Layout> sl = plan_layouts(sneed, n, ms, lows, c64, 1 << 30, &srot, kMaxSweepSlots);
+    std::vector<int> sord;   // sweep entries (entry e is gate n_gates - 1 - e) pass by pass
+    std::vector<Layout> sl = plan_layouts_ahead(sneed, sdl, n, ms, lows, c64, 1 << 30, &srot, kMaxSweepSlots, sord);
     std::vector<BOp> sops(std::max(1, n_gates));
     std::vector<SweepOp<R>> rsops(std::max(1, n_gates));
     std::memset(rsops.data(), 0, sizeof(SweepOp<R>) * rsops.size());
@@ -665,10 +751,17 @@ void build_group(Group &g, const Tuning &t) {
     for (const Layout &l : sl) {
         int local[64];
         BPass p = make_pass(l, local);
+        if (sregs) {   // diagonal gates where none of their lanes is a register bit
+            constexpr int A = (1 << C::SweepBits) / kBatchRegThreads;
+            place_diagonals(
+                sord, l.begin, l.end, local, p.m, Log2<A>::v, Log2<kBatchRegThreads / 64>::v,
+                [&](int e) { return sneed[e] == 0; }, [&](int e) { return g.ops[n_gates - 1 - e].target; },
+                [&](int e) { return lanes_of(n_gates - 1 - e); });
+        }
         g.spass_slot_begin.push_back((int)slot_out.size());
         int slots = 0;
         for (int i = l.begin; i < l.end; ++i) {
-            const int gi = n_gates - 1 - i;
+            const int gi = n_gates - 1 - sord[i];
             const CircuitOp &op = g.ops[gi];
             BOp &o = sops[i];
             o.gctrl = 0;
@@ -679,7 +772,7 @@ void build_group(Group &g, const Tuning &t) {
                 else
                     o.gctrl |= u64(1) << c;
             }
-            const bool diag = sneed[i] == 0;
+            const bool diag = sneed[sord[i]] == 0;
             o.kind = (short)(local[op.target] >= 0 ? (diag ? kOpDiagLocal : kOpPair) : kOpDiagOuter);
             o.target = (short)(local[op.target] >= 0 ? local[op.target] : op.target);
             o.gate = gi;

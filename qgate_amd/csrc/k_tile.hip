@@ -667,79 +667,16 @@ void adjoint_sweep(Cx<R> *phi, Cx<R> *lam, int n_lanes, const std::vector<SweepG
         int local[64];
         for (int b = 0; b < 64; ++b) local[b] = -1;
         for (int j = 0; j < ps.m; ++j) local[bits[j]] = j;
-        if (regs && ps.m == C::RegBits) {
-            // A register pass keeps its pair updates in order, and puts each diagonal gate at a
-            // position, between the pair updates on its lanes before and after it, whose phase
-            // has none of its tile lanes as a register bit: there it is a per-thread factor, not
-            // an update of both register arrays. (A pass that takes the gates of several
-            // layers otherwise meets a layer's RZ and CZ right after the RY that made their
-            // lanes register bits: the 24-qubit ansatz's complex64 sweep did 253 such updates
-            // instead of 58 and ran 5% slower in half the passes.) Diagonal gates commute with
-            // each other and with every pair update whose target is none of their lanes, as
-            // do their generators, so the states and the gradient terms are unchanged.
-            auto is_diag = [&](const SweepGate &g) { return is_diagonal(g.udag) && (g.pauli == 0 || g.pauli == 3); };
-            std::vector<int> pairs, pair_local;
-            for (int i = pl.begin; i < pl.end; ++i)
-                if (!is_diag(gates[order[i]])) {
-                    pairs.push_back(order[i]);
-                    pair_local.push_back(local[gates[order[i]].target]);
-                }
-            const int np = (int)pairs.size();
-            if (np > 0) {
-                const std::vector<PhaseLayout> L =
-                    plan_phase_layouts(pair_local, ps.m, Log2<A>::v, Log2<C::RegThreads / 64>::v);
-                std::vector<int> phase_of(np, 0);
-                for (int ph = 0; ph < (int)L.size(); ++ph)
-                    for (int k = L[ph].begin; k < L[ph].end; ++k) phase_of[k] = ph;
-                // a gate after pair update gp - 1 (before the first: gp = 0) runs in its phase
-                auto gap_phase = [&](int gp) { return gp >= 1 ? phase_of[gp - 1] : 0; };
-                std::vector<std::vector<int>> at(np + 1);
-                int seen = 0;
-                for (int i = pl.begin; i < pl.end; ++i) {
-                    const int gi = order[i];
-                    const SweepGate &g = gates[gi];
-                    if (!is_diag(g)) {
-                        ++seen;
-                        continue;
-                    }
-                    std::vector<int> lanes(g.ctrl);
-                    lanes.push_back(g.target);
-                    auto hits = [&](int k) {
-                        return std::find(lanes.begin(), lanes.end(), gates[pairs[k]].target) != lanes.end();
-                    };
-                    int lo = 0, hi = np;
-                    for (int k = seen - 1; k >= 0; --k)
-                        if (hits(k)) {
-                            lo = k + 1;
-                            break;
-                        }
-                    for (int k = seen; k < np; ++k)
-                        if (hits(k)) {
-                            hi = k;
-                            break;
-                        }
-                    auto bad = [&](int gp) {
-                        const std::vector<int> &rg = L[gap_phase(gp)].regs;
-                        for (int l : lanes)
-                            if (local[l] >= 0 && std::find(rg.begin(), rg.end(), local[l]) != rg.end()) return true;
-                        return false;
-                    };
-                    int best = seen;
-                    for (int d = 1; bad(best) && (seen - d >= lo || seen + d <= hi); ++d) {
-                        if (seen - d >= lo && !bad(seen - d))
-                            best = seen - d;
-                        else if (seen + d <= hi && !bad(seen + d))
-                            best = seen + d;
-                    }
-                    at[best].push_back(gi);
-                }
-                int w = pl.begin;
-                for (int gp = 0; gp <= np; ++gp) {
-                    for (int gi : at[gp]) order[w++] = gi;
-                    if (gp < np) order[w++] = pairs[gp];
-                }
-            }
-        }
+        if (regs && ps.m == C::RegBits)   // diagonal gates where none of their lanes is a register bit
+            place_diagonals(
+                order, pl.begin, pl.end, local, ps.m, Log2<A>::v, Log2<C::RegThreads / 64>::v,
+                [&](int gi) { return is_diagonal(gates[gi].udag) && (gates[gi].pauli == 0 || gates[gi].pauli == 3); },
+                [&](int gi) { return gates[gi].target; },
+                [&](int gi) {
+                    std::vector<int> l(gates[gi].ctrl);
+                    l.push_back(gates[gi].target);
+                    return l;
+                });
         int slots = 0;
         for (int i = pl.begin; i < pl.end; ++i) {
             const SweepGate &g = gates[order[i]];

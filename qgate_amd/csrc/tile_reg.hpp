@@ -503,6 +503,75 @@ inline std::vector<PhaseLayout> plan_phase_layouts(const std::vector<int> &pair,
     return out;
 }
 
+// A register pass keeps its pair updates in order and puts each diagonal gate at a position,
+// between the pair updates on its lanes before and after it, whose phase has none of its tile
+// lanes as a register bit: there it is a per-thread factor, not an update of the register
+// arrays. (A pass that takes the gates of several layers otherwise meets a layer's RZ and CZ
+// right after the RY that made their lanes register bits.) Diagonal gates commute with each
+// other and with every pair update whose target is none of their lanes, as do the generators
+// of diagonal rotations, so the states and gradient terms are unchanged. order[begin, end) are
+// the pass's gates; is_diag(g), target_of(g) and lanes_of(g) (target and controls) describe
+// gate g; local maps a state lane to its tile bit (-1 outside the tile).
+template <class IsDiag, class TargetOf, class LanesOf>
+inline void place_diagonals(std::vector<int> &order, int begin, int end, const int *local, int m, int rb_count,
+                            int wb_count, IsDiag is_diag, TargetOf target_of, LanesOf lanes_of) {
+    std::vector<int> pairs, pair_local;
+    for (int i = begin; i < end; ++i)
+        if (!is_diag(order[i])) {
+            pairs.push_back(order[i]);
+            pair_local.push_back(local[target_of(order[i])]);
+        }
+    const int np = (int)pairs.size();
+    if (np == 0) return;
+    const std::vector<PhaseLayout> L = plan_phase_layouts(pair_local, m, rb_count, wb_count);
+    std::vector<int> phase_of(np, 0);
+    for (int ph = 0; ph < (int)L.size(); ++ph)
+        for (int k = L[ph].begin; k < L[ph].end; ++k) phase_of[k] = ph;
+    // a gate after pair update gp - 1 (before the first: gp = 0) runs in its phase
+    auto gap_phase = [&](int gp) { return gp >= 1 ? phase_of[gp - 1] : 0; };
+    std::vector<std::vector<int>> at(np + 1);
+    int seen = 0;
+    for (int i = begin; i < end; ++i) {
+        const int gi = order[i];
+        if (!is_diag(gi)) {
+            ++seen;
+            continue;
+        }
+        const std::vector<int> lanes = lanes_of(gi);
+        auto hits = [&](int k) { return std::find(lanes.begin(), lanes.end(), target_of(pairs[k])) != lanes.end(); };
+        int lo = 0, hi = np;
+        for (int k = seen - 1; k >= 0; --k)
+            if (hits(k)) {
+                lo = k + 1;
+                break;
+            }
+        for (int k = seen; k < np; ++k)
+            if (hits(k)) {
+                hi = k;
+                break;
+            }
+        auto bad = [&](int gp) {
+            const std::vector<int> &rg = L[gap_phase(gp)].regs;
+            for (int l : lanes)
+                if (local[l] >= 0 && std::find(rg.begin(), rg.end(), local[l]) != rg.end()) return true;
+            return false;
+        };
+        int best = seen;
+        for (int d = 1; bad(best) && (seen - d >= lo || seen + d <= hi); ++d) {
+            if (seen - d >= lo && !bad(seen - d))
+                best = seen - d;
+            else if (seen + d <= hi && !bad(seen + d))
+                best = seen + d;
+        }
+        at[best].push_back(gi);
+    }
+    int w = begin;
+    for (int gp = 0; gp <= np; ++gp) {
+        for (int gi : at[gp]) order[w++] = gi;
+        if (gp < np) order[w++] = pairs[gp];
+    }
+}
+
 // Packed form of a phase layout (ops [begin, end) of its pass).
 inline PhaseBits phase_bits(const PhaseLayout &P) {
     PhaseBits h;
