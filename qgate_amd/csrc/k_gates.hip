@@ -1046,14 +1046,15 @@ void apply_dense_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &
     // matrices of the multi-gate blocks, uploaded together
     std::vector<size_t> offset(blocks.size(), 0);
     size_t total = 0;
+    // (a block on one lane is a 2x2 gate, streamed by launch_gate below: no matrix upload)
     for (size_t b = 0; b < blocks.size(); ++b) {
-        if (blocks[b].gates.size() < 2) continue;
+        if (blocks[b].gates.size() < 2 || blocks[b].lanes.size() < 2) continue;
         offset[b] = total;
         total += (unitary_bytes<R>((int)blocks[b].lanes.size(), n_lanes) + 255) & ~size_t(255);
     }
     std::vector<char> host(std::max<size_t>(total, 1));
     for (size_t b = 0; b < blocks.size(); ++b)
-        if (blocks[b].gates.size() >= 2)
+        if (blocks[b].gates.size() >= 2 && blocks[b].lanes.size() >= 2)
             pack_unitary<R>((int)blocks[b].lanes.size(), n_lanes, block_unitary(blocks[b].lanes, blocks[b].gates),
                             host.data() + offset[b]);
     char *dev = total > 0 ? static_cast<char *>(dev_alloc(total)) : nullptr;
@@ -1065,6 +1066,12 @@ void apply_dense_sequence(Cx<R> *a, int n_lanes, const std::vector<QueuedGate> &
             if (bl.gates.size() == 1) {
                 const QueuedGate &g = *bl.gates[0];
                 launch_gate<R>(a, n_lanes, g.m, g.ctrl, g.target);
+            } else if (bl.lanes.size() == 1) {   // the product of the block's gates, in FP64
+                const std::vector<std::complex<double>> u = block_unitary(bl.lanes, bl.gates);
+                GateMatrix gm;
+                for (int r = 0; r < 2; ++r)
+                    for (int c = 0; c < 2; ++c) gm.m[r][c] = u[(size_t)r * 2 + c];
+                launch_gate<R>(a, n_lanes, gm, std::vector<int>(), bl.lanes[0]);
             } else {
                 launch_unitary_dev<R>(a, n_lanes, bl.lanes, dev + offset[b]);
             }
